@@ -1,0 +1,147 @@
+"""ctypes view of the C ABI in include/evo_engine.h and include/evo_types.h.
+
+The structs mirror the C layouts field for field; `check_layouts()` compares
+their sizes with the C side's static assertions (164 B params, 76 B result).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "_lib" / "libevo_gpu.so"
+
+EVO_OK = 0
+EVO_ERR_INVALID = -1
+EVO_ERR_OOM = -2
+EVO_ERR_DEVICE = -3
+EVO_ERR_BUSY = -4
+EVO_ERR_NOT_READY = -5
+
+MAX_NODES = 32
+MAX_MUSCLES = 96
+NODE_WORDS = 6
+BONE_WORDS = 6
+MUSCLE_WORDS = 20
+NONE = 0xFFFFFFFF
+
+# World effect bits (src/kernel.rs:82-113).
+GROUND, TERRAIN, SLOPE, GAPS, HURDLES, QUAKE, MUD, WATER, ICE, WIND, AIR, BRAMBLES = (1 << i for i in range(12))
+FLAG_NAMES = ["GROUND", "TERRAIN", "SLOPE", "GAPS", "HURDLES", "QUAKE", "MUD", "WATER", "ICE", "WIND", "AIR",
+              "BRAMBLES"]
+
+
+class Rung(C.Structure):
+    _fields_ = [("w", C.c_float * 6), ("bias", C.c_float), ("off", C.c_uint32)]
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("count", C.c_uint32), ("base", C.c_uint32), ("steps", C.c_uint32), ("screen_step", C.c_uint32),
+        ("stride", C.c_uint32), ("gravity", C.c_float), ("air", C.c_float), ("friction", C.c_float),
+        ("terrain", C.c_float), ("muscle_energy", C.c_float), ("muscle_recovery", C.c_float),
+        ("slope", C.c_float), ("wind", C.c_float), ("mud", C.c_float), ("gaps", C.c_float),
+        ("hurdles", C.c_float), ("quake", C.c_float), ("screen_bar", C.c_float),
+        ("screen_bar_young", C.c_float), ("screen_bar_reshaped", C.c_float), ("water", C.c_float),
+        ("patches", C.c_float), ("air_sub", C.c_float), ("inv_muscle_energy", C.c_float),
+        ("brambles", C.c_float), ("r1", Rung), ("r2", Rung),
+    ]
+
+
+RESULT_FIELDS = [
+    "fitness", "ground_contact", "vertical_oscillation", "gait_frequency", "previous_center_y",
+    "vertical_extremum", "vertical_trend", "gait_turns", "height_sum", "contact_lo", "contact_hi", "lift_lo",
+    "lift_hi", "ground_lo", "ground_hi", "fall_time", "head_shake", "screen_x", "screened",
+]
+
+
+class Result(C.Structure):
+    _fields_ = [(name, C.c_float) for name in RESULT_FIELDS]
+
+
+class Run(C.Structure):
+    _fields_ = [("flags", C.c_uint32), ("rate", C.c_uint32), ("substeps", C.c_uint32), ("record", C.c_uint32),
+                ("max_nodes", C.c_uint32), ("max_muscles", C.c_uint32)]
+
+
+class WorldConfig(C.Structure):
+    _fields_ = [
+        ("duration", C.c_float), ("gravity", C.c_float), ("air_retention", C.c_float),
+        ("ground_friction", C.c_float), ("ground", C.c_int32), ("terrain", C.c_float),
+        ("muscle_energy", C.c_float), ("muscle_recovery", C.c_float), ("slope", C.c_float), ("wind", C.c_float),
+        ("mud", C.c_float), ("brambles", C.c_float), ("water", C.c_float), ("patches", C.c_float),
+        ("gaps", C.c_float), ("hurdles", C.c_float), ("quake", C.c_float), ("rate", C.c_uint32),
+        ("substeps", C.c_uint32), ("screen_seconds", C.c_float), ("screen_bar", C.c_float),
+        ("screen_bar_young", C.c_float), ("screen_bar_reshaped", C.c_float), ("r1", Rung), ("r2", Rung),
+    ]
+
+
+class SynthSpec(C.Structure):
+    _fields_ = [
+        ("seed", C.c_uint64), ("count", C.c_uint32), ("min_nodes", C.c_uint32), ("max_nodes", C.c_uint32),
+        ("node_p", C.c_float), ("min_muscles", C.c_uint32), ("max_muscles", C.c_uint32),
+        ("max_constraints", C.c_uint32), ("mass", C.c_float), ("radius", C.c_float), ("friction_lo", C.c_float),
+        ("friction_hi", C.c_float), ("bone_lo", C.c_float), ("bone_hi", C.c_float), ("short_lo", C.c_float),
+        ("short_hi", C.c_float), ("long_lo", C.c_float), ("long_hi", C.c_float), ("period_lo", C.c_float),
+        ("period_hi", C.c_float), ("strength_lo", C.c_float), ("strength_hi", C.c_float), ("duty", C.c_float),
+        ("stiffness", C.c_float), ("joint_lo", C.c_float), ("joint_hi", C.c_float), ("sorted", C.c_uint32),
+        ("id_base", C.c_uint32), ("world", WorldConfig),
+    ]
+
+
+RESULT_DTYPE = np.dtype([(name, np.float32) for name in RESULT_FIELDS])
+assert RESULT_DTYPE.itemsize == 76
+
+_P = C.c_void_p
+_u32 = C.c_uint32
+_u64 = C.c_uint64
+
+# Exported symbols with their signatures (argtypes, restype).
+SIGNATURES = {
+    "evo_world_default": ([C.POINTER(WorldConfig)], None),
+    "evo_world_flags": ([C.POINTER(WorldConfig)], _u32),
+    "evo_world_params": ([C.POINTER(WorldConfig), _u32, _u32, _u32, C.POINTER(Params), C.POINTER(Run)], C.c_int),
+    "evo_settle_frames": ([_u32], _u32),
+    "evo_quake_hash": ([_u64], _u32),
+    "evo_ground_height": ([C.POINTER(WorldConfig), C.c_float, C.c_float, C.c_float], C.c_float),
+    "evo_synth_preset": ([C.c_int, C.POINTER(SynthSpec)], C.c_int),
+    "evo_synth_sizes": ([C.POINTER(SynthSpec), C.POINTER(_u64), C.POINTER(_u64)], C.c_int),
+    "evo_synth_fill": ([C.POINTER(SynthSpec), _P, _P, _P, _P, _P, _P], C.c_int),
+    "evo_advance_device": ([_P, _P, _P, _P, _P, C.POINTER(Params), C.POINTER(Run), _P, _P], C.c_int),
+    "evo_validate_batch": ([_P, _u64, _P, _u64, _P, _u32, C.POINTER(_u32)], C.c_int),
+    "evo_max_muscles": ([_P, _u32, _u32], _u32),
+    "evo_build_info": ([], C.c_char_p),
+}
+
+_lib: C.CDLL | None = None
+
+
+def check_layouts() -> None:
+    assert C.sizeof(Rung) == 32
+    assert C.sizeof(Params) == 164
+    assert C.sizeof(Result) == 76
+    assert C.sizeof(Run) == 24
+
+
+def lib() -> C.CDLL:
+    """The loaded library. Raises when it is not built: there is no fallback."""
+    global _lib
+    if _lib is None:
+        path = Path(os.environ.get("EVO_LIB", LIB_PATH))
+        if not path.exists():
+            raise RuntimeError(f"{path} is missing: build it with `python -m evolution_simulator_gpu_amd.build`")
+        handle = C.CDLL(str(path))
+        for name, (argtypes, restype) in SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.argtypes = argtypes
+            fn.restype = restype
+        check_layouts()
+        _lib = handle
+    return _lib
+
+
+def ptr(a: np.ndarray | None) -> int | None:
+    return None if a is None else a.ctypes.data

@@ -1,0 +1,181 @@
+"""Packed batches, worlds and the device-resident simulator call.
+
+A `Batch` holds the three buffers of the reference's `WavePack`
+(src/kernel.rs:56-72): node and bone records, muscle records and two head
+words per creature, plus `slots`, the caller's index of each packed creature
+(src/creature_kernel.rs:280-310). `simulate` runs one on the GPU through
+`evo_advance_device`; torch provides the device memory and the stream.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _native as N
+
+
+@dataclass
+class Batch:
+    records: np.ndarray  # uint32
+    muscles: np.ndarray  # float32
+    heads: np.ndarray    # uint32, shape (count, 8)
+    slots: np.ndarray    # uint32, caller index of each packed creature
+
+    @property
+    def count(self) -> int:
+        return int(self.heads.shape[0])
+
+    def nodes(self) -> np.ndarray:
+        return self.heads[:, 0] & 255
+
+    def muscle_counts(self) -> np.ndarray:
+        return self.heads[:, 1]
+
+    def validate(self) -> None:
+        bad = C.c_uint32(0)
+        rc = N.lib().evo_validate_batch(N.ptr(self.records), self.records.size, N.ptr(self.muscles),
+                                        self.muscles.size, N.ptr(self.heads), self.count, C.byref(bad))
+        if rc != N.EVO_OK:
+            raise ValueError(f"creature {bad.value} of the batch cannot be simulated")
+
+    def subset(self, index: np.ndarray) -> "Batch":
+        """The creatures at `index` (packed positions) as a batch of their own."""
+        index = np.asarray(index, dtype=np.int64)
+        recs, muss, heads = [], [], []
+        rw = mf = 0
+        for c in index:
+            h = self.heads[c].copy()
+            n, m = int(h[0] & 255), int(h[1])
+            words = n * N.NODE_WORDS + (n - 1) * N.BONE_WORDS
+            recs.append(self.records[h[3]:h[3] + words])
+            muss.append(self.muscles[h[4]:h[4] + m * N.MUSCLE_WORDS])
+            h[3], h[4] = rw, mf
+            rw += words
+            mf += m * N.MUSCLE_WORDS
+            heads.append(h)
+        return Batch(np.concatenate(recs).astype(np.uint32) if recs else np.zeros(1, np.uint32),
+                     np.concatenate(muss).astype(np.float32) if mf else np.zeros(4, np.float32),
+                     np.stack(heads).astype(np.uint32), self.slots[index].astype(np.uint32))
+
+
+def world(**overrides) -> N.WorldConfig:
+    """The calm default world with `overrides` applied (src/config.rs:118-162)."""
+    w = N.WorldConfig()
+    N.lib().evo_world_default(C.byref(w))
+    for key, value in overrides.items():
+        if key in ("r1", "r2"):
+            setattr(w, key, value)
+        else:
+            setattr(w, key, value)
+    return w
+
+
+def params_for(w: N.WorldConfig, base: int, count: int, stride: int = 0) -> tuple[N.Params, N.Run]:
+    p, run = N.Params(), N.Run()
+    rc = N.lib().evo_world_params(C.byref(w), base, count, stride, C.byref(p), C.byref(run))
+    if rc != N.EVO_OK:
+        raise ValueError("invalid world settings")
+    return p, run
+
+
+def synth_spec(preset: int, count: int | None = None, **overrides) -> N.SynthSpec:
+    s = N.SynthSpec()
+    if N.lib().evo_synth_preset(preset, C.byref(s)) != N.EVO_OK:
+        raise ValueError(f"unknown preset {preset}")
+    if count is not None:
+        s.count = count
+    for key, value in overrides.items():
+        setattr(s, key, value)
+    return s
+
+
+def synth(spec: N.SynthSpec, worlds: list[N.WorldConfig] | None = None,
+          world_of: np.ndarray | None = None) -> Batch:
+    rw, mf = C.c_uint64(0), C.c_uint64(0)
+    lib = N.lib()
+    if lib.evo_synth_sizes(C.byref(spec), C.byref(rw), C.byref(mf)) != N.EVO_OK:
+        raise ValueError("invalid synthetic spec")
+    records = np.zeros(max(rw.value, 1), np.uint32)
+    muscles = np.zeros(max(mf.value, 4), np.float32)
+    heads = np.zeros((spec.count, 8), np.uint32)
+    slots = np.zeros(spec.count, np.uint32)
+    warr = None
+    wof = None
+    if worlds is not None and world_of is not None:
+        warr = (N.WorldConfig * len(worlds))(*worlds)
+        wof = np.ascontiguousarray(world_of, dtype=np.uint32)
+    rc = lib.evo_synth_fill(C.byref(spec), N.ptr(records), N.ptr(muscles), N.ptr(heads), N.ptr(slots),
+                            C.cast(warr, C.c_void_p) if warr is not None else None, N.ptr(wof))
+    if rc != N.EVO_OK:
+        raise ValueError("synthetic fill failed")
+    return Batch(records, muscles, heads, slots)
+
+
+def frame_stride(batch: Batch) -> int:
+    """Float pairs per recorded frame (src/creature_kernel.rs:323-326)."""
+    return 2 * N.MAX_NODES + int(batch.muscle_counts().max(initial=0)) + 1
+
+
+def run_for(batch: Batch, w: N.WorldConfig, record: bool = False) -> tuple[N.Params, N.Run]:
+    p, run = params_for(w, 0, batch.count, frame_stride(batch))
+    run.record = 1 if record else 0
+    run.max_nodes = int(batch.nodes().max(initial=0))
+    run.max_muscles = int(batch.muscle_counts().max(initial=0))
+    return p, run
+
+
+class DeviceBatch:
+    """A batch resident in device memory (torch tensors on `device`)."""
+
+    def __init__(self, batch: Batch, device: str = "cuda"):
+        import torch
+
+        self.batch = batch
+        self.device = device
+        self.records = torch.from_numpy(batch.records.view(np.int32)).to(device)
+        self.muscles = torch.from_numpy(batch.muscles).to(device)
+        self.heads = torch.from_numpy(batch.heads.reshape(-1).view(np.int32)).to(device)
+        self.results = torch.zeros(batch.count * 19, dtype=torch.float32, device=device)
+        self.counter = torch.zeros(4, dtype=torch.int32, device=device)
+        self.frames = None
+
+    def alloc_frames(self, p: N.Params, rate: int) -> None:
+        import torch
+
+        fpc = N.lib().evo_settle_frames(rate) + p.steps + 1
+        self.frames = torch.zeros(self.batch.count * fpc * p.stride * 2, dtype=torch.float32, device=self.device)
+
+    def launch(self, p: N.Params, run: N.Run, stream=None) -> None:
+        import torch
+
+        if stream is None:
+            stream = torch.cuda.current_stream(self.records.device)
+        rc = N.lib().evo_advance_device(
+            self.records.data_ptr(), self.muscles.data_ptr(), self.heads.data_ptr(), self.results.data_ptr(),
+            self.counter.data_ptr(), C.byref(p), C.byref(run),
+            self.frames.data_ptr() if self.frames is not None else None, C.c_void_p(stream.cuda_stream))
+        if rc != N.EVO_OK:
+            raise RuntimeError(f"evo_advance_device failed: {rc}")
+
+    def results_numpy(self) -> np.ndarray:
+        return self.results.cpu().numpy().view(N.RESULT_DTYPE).reshape(-1)
+
+
+def simulate(batch: Batch, w: N.WorldConfig, record: bool = False, device: str = "cuda"):
+    """Runs every creature of `batch` in world `w` on the GPU. Returns the
+    results in packed order (and the frames when recording)."""
+    import torch
+
+    batch.validate()
+    p, run = run_for(batch, w, record)
+    db = DeviceBatch(batch, device)
+    if record:
+        db.alloc_frames(p, w.rate)
+    db.launch(p, run)
+    torch.cuda.synchronize()
+    res = db.results_numpy()
+    if record:
+        return res, db.frames.cpu().numpy()
+    return res

@@ -1,0 +1,92 @@
+"""Builds the simulator library for gfx950 in-tree.
+
+`build()` compiles the HIP kernels (one translation unit per step rate and
+substep count, in parallel) and the host C++ files with hipcc, and links
+`evolution_simulator_gpu_amd/_lib/libevo_gpu.so`. It always compiles, even
+when a built library exists.
+"""
+from __future__ import annotations
+
+import concurrent.futures as futures
+import os
+import subprocess
+import sys
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+CSRC = PKG / "csrc"
+INCLUDE = PKG.parent / "include"
+OBJ = PKG / "_build"
+LIB_DIR = PKG / "_lib"
+LIB = LIB_DIR / "libevo_gpu.so"
+
+ARCH = "gfx950"
+# Contraction off (every fused multiply-add is written out), IEEE division
+# and square root, and no packed FP32 instructions.
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-ffp-contract=off",
+    "-fhip-fp32-correctly-rounded-divide-sqrt",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall",
+    "-Wno-unused-function",
+    f"-I{INCLUDE}",
+]
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", f"-I{INCLUDE}"]
+
+RATES_SUBSTEPS = [(60, 8), (60, 16), (120, 8), (120, 16)]
+HOST_SOURCES = ["world.cpp", "synth.cpp", "batch.cpp", "half.cpp"]
+HIP_SOURCES = ["advance.hip"]
+
+
+def _hipcc() -> str:
+    rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
+    cand = Path(rocm) / "bin" / "hipcc"
+    return str(cand) if cand.exists() else "hipcc"
+
+
+def _run(cmd: list[str]) -> str:
+    proc = subprocess.run(cmd, capture_output=True, text=True)
+    if proc.returncode != 0:
+        raise RuntimeError(f"command failed ({proc.returncode}): {' '.join(cmd)}\n{proc.stdout}\n{proc.stderr}")
+    return proc.stderr
+
+
+def _jobs() -> list[tuple[list[str], Path]]:
+    hipcc = _hipcc()
+    jobs = []
+    for rate, sub in RATES_SUBSTEPS:
+        out = OBJ / f"instances_r{rate}_s{sub}.o"
+        jobs.append(([hipcc, *HIP_FLAGS, f"-DEVO_RATE={rate}", f"-DEVO_SUB={sub}", "-c",
+                      str(CSRC / "instances.hip"), "-o", str(out)], out))
+    for src in HIP_SOURCES:
+        out = OBJ / (Path(src).stem + ".o")
+        jobs.append(([hipcc, *HIP_FLAGS, "-c", str(CSRC / src), "-o", str(out)], out))
+    for src in HOST_SOURCES:
+        out = OBJ / (Path(src).stem + ".o")
+        jobs.append((["g++", *HOST_FLAGS, "-c", str(CSRC / src), "-o", str(out)], out))
+    return jobs
+
+
+def build(jobs: int | None = None, verbose: bool = False) -> Path:
+    """Compiles every kernel and host file and links the library."""
+    OBJ.mkdir(parents=True, exist_ok=True)
+    LIB_DIR.mkdir(parents=True, exist_ok=True)
+    work = _jobs()
+    workers = jobs or min(8, os.cpu_count() or 1)
+    with futures.ThreadPoolExecutor(max_workers=workers) as pool:
+        for log in pool.map(lambda j: _run(j[0]), work):
+            if verbose and log:
+                sys.stderr.write(log)
+    tmp = LIB.with_suffix(".so.tmp")
+    _run([_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *[str(o) for _, o in work],
+          "-lpthread"])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

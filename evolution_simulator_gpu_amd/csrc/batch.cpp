@@ -1,0 +1,58 @@
+// Host checks of a packed batch before it reaches the device. The reference
+// checks body sizes when it packs (src/kernel.rs:477-486); a batch handed to
+// this library may come from anywhere, so every index the kernel follows is
+// checked here, and the kernel itself trusts the batch.
+#include <algorithm>
+
+#include "host_common.hpp"
+
+extern "C" {
+
+int evo_validate_batch(const uint32_t* records, uint64_t record_words, const float* muscles, uint64_t muscle_floats,
+                       const uint32_t* heads, uint32_t creatures, uint32_t* bad) {
+    for (uint32_t c = 0; c < creatures; c++) {
+        const uint32_t* h = heads + 8ull * c;
+        const uint32_t nodes = h[0] & 255u, nm = h[1];
+        auto fail = [&]() {
+            if (bad) { *bad = c; }
+            return EVO_ERR_INVALID;
+        };
+        if (h[0] > 255u || nodes < 2u || nodes > EVO_MAX_NODES || nm > EVO_MAX_MUSCLES) { return fail(); }
+        const uint64_t words = static_cast<uint64_t>(nodes) * EVO_NODE_WORDS +
+                               static_cast<uint64_t>(nodes - 1u) * EVO_BONE_WORDS;
+        if (static_cast<uint64_t>(h[3]) + words > record_words) { return fail(); }
+        if (static_cast<uint64_t>(h[4]) + static_cast<uint64_t>(nm) * EVO_MUSCLE_WORDS > muscle_floats) { return fail(); }
+        if (h[4] % 4u != 0u) { return fail(); }  // muscle lines are read as 16-byte vectors
+        const uint32_t* rec = records + h[3];
+        for (uint32_t i = 0; i < nodes; i++) {
+            const float mass = evo::as_float(rec[i * EVO_NODE_WORDS]);
+            const float radius = evo::as_float(rec[i * EVO_NODE_WORDS + 1]);
+            if (!(mass > 0.0f) || !(radius > 0.0f)) { return fail(); }
+        }
+        const uint32_t* bones = rec + nodes * EVO_NODE_WORDS;
+        for (uint32_t j = 0; j + 1u < nodes; j++) {
+            const uint32_t* b = bones + j * EVO_BONE_WORDS;
+            if (b[0] > j) { return fail(); }
+            if (j > 0 && b[2] != EVO_NONE && b[2] >= j) { return fail(); }
+        }
+        const float* m = muscles + h[4];
+        for (uint32_t k = 0; k < nm; k++) {
+            const uint32_t n = evo::as_uint(m[k * EVO_MUSCLE_WORDS]);
+            if ((n & 255u) >= nodes || ((n >> 8) & 255u) >= nodes || ((n >> 16) & 255u) >= nodes ||
+                (n >> 24) >= nodes) {
+                return fail();
+            }
+            const uint32_t sensor = evo::as_uint(m[k * EVO_MUSCLE_WORDS + 16]);
+            if (sensor != EVO_NONE && sensor >= nodes) { return fail(); }
+        }
+    }
+    return EVO_OK;
+}
+
+uint32_t evo_max_muscles(const uint32_t* heads, uint32_t base, uint32_t count) {
+    uint32_t m = 0;
+    for (uint32_t c = base; c < base + count; c++) { m = std::max(m, heads[8ull * c + 1]); }
+    return m;
+}
+
+}  // extern "C"

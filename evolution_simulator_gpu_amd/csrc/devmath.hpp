@@ -1,0 +1,118 @@
+// Elementary functions of the simulator, for device and host code.
+//
+// Written from published methods, not from any vendor math library:
+//   cos_0_pi   cos on [0, pi] as -sin(x - pi/2) with the Cody-Waite
+//              two-constant reduction (Cody & Waite, 1980) and a
+//              least-squares polynomial on Chebyshev nodes for
+//              sin(y) = y + y^3 P(y^2), |y| <= pi/2 + 0.02; max error 1.3e-7.
+//   atan2f_evo octant reduction to a = min/max in [0, 1], atan(a) = a Q(a^2)
+//              (least squares on Chebyshev nodes, max error 1.1e-7), then the
+//              quadrant fix-ups with IEEE 754 sign rules.
+//   f2h / h2f  IEEE binary16, round to nearest even, with the semantics of
+//              src/creature_kernel.rs:221-274 (the reference's kernel uses
+//              its vendor's conversion instruction, shaders/creature.cu:137-147).
+// Every step is an IEEE operation (fmaf is one fused multiply-add), so the
+// results do not depend on the processor that evaluates them.
+#pragma once
+
+#include <cstdint>
+
+#ifdef __HIPCC__
+#define EVO_HD __host__ __device__ __forceinline__
+#else
+#define EVO_HD inline
+#endif
+
+namespace evo {
+
+EVO_HD float bits_f(uint32_t u) { return __builtin_bit_cast(float, u); }
+EVO_HD uint32_t f_bits(float f) { return __builtin_bit_cast(uint32_t, f); }
+
+EVO_HD float cos_0_pi(float x) {
+    const float c1 = 0x1.921fb6p+0f;
+    const float c2 = -0x1.777a5cp-25f;
+    const float y = (x - c1) - c2;
+    const float z = y * y;
+    float p = -0x1.9914fcp-26f;
+    p = __builtin_fmaf(p, z, 0x1.715fc4p-19f);
+    p = __builtin_fmaf(p, z, -0x1.a01744p-13f);
+    p = __builtin_fmaf(p, z, 0x1.11110ap-7f);
+    p = __builtin_fmaf(p, z, -0x1.555556p-3f);
+    const float s = __builtin_fmaf(y * z, p, y);
+    return -s;
+}
+
+EVO_HD float atan_unit(float a) {
+    const float z = a * a;
+    float q = -0x1.8d8782p-10f;
+    q = __builtin_fmaf(q, z, 0x1.3ac9dcp-7f);
+    q = __builtin_fmaf(q, z, -0x1.d44950p-6f);
+    q = __builtin_fmaf(q, z, 0x1.c3bef6p-5f);
+    q = __builtin_fmaf(q, z, -0x1.50cd90p-4f);
+    q = __builtin_fmaf(q, z, 0x1.bdbc88p-4f);
+    q = __builtin_fmaf(q, z, -0x1.23cec0p-3f);
+    q = __builtin_fmaf(q, z, 0x1.9987eap-3f);
+    q = __builtin_fmaf(q, z, -0x1.5554f8p-2f);
+    q = __builtin_fmaf(q, z, 1.0f);
+    return a * q;
+}
+
+EVO_HD float atan2f_evo(float y, float x) {
+    const float ax = __builtin_fabsf(x), ay = __builtin_fabsf(y);
+    const float mx = __builtin_fmaxf(ax, ay), mn = __builtin_fminf(ax, ay);
+    float a = mn / mx;
+    if (mx == 0.0f) { a = 0.0f; }
+    if (mx == __builtin_inff() && mn == __builtin_inff()) { a = 1.0f; }
+    float t = atan_unit(a);
+    if (ay > ax) { t = 0x1.921fb6p+0f - t; }
+    if (__builtin_signbit(x)) { t = 0x1.921fb6p+1f - t; }
+    if (x != x || y != y) { t = x + y; }
+    return __builtin_copysignf(t, y);
+}
+
+EVO_HD uint32_t f2h(float v) {
+    const uint32_t bits = f_bits(v);
+    const uint32_t sign = (bits >> 16) & 0x8000u;
+    const int32_t exponent = static_cast<int32_t>((bits >> 23) & 0xffu);
+    const uint32_t mantissa = bits & 0x7fffffu;
+    if (exponent == 0xff) { return sign | 0x7c00u | (mantissa != 0u ? 0x200u : 0u); }
+    const int32_t e = exponent - 127 + 15;
+    if (e >= 31) { return sign | 0x7c00u; }
+    if (e <= 0) {
+        if (e < -10) { return sign; }
+        const uint32_t m = mantissa | 0x800000u;
+        const uint32_t shift = static_cast<uint32_t>(14 - e);
+        const uint32_t half = 1u << (shift - 1u);
+        const uint32_t rest = m & ((1u << shift) - 1u);
+        uint32_t out = m >> shift;
+        if (rest > half || (rest == half && (out & 1u) == 1u)) { out += 1u; }
+        return sign | out;
+    }
+    const uint32_t rest = mantissa & 0x1fffu;
+    uint32_t out = (static_cast<uint32_t>(e) << 10) | (mantissa >> 13);
+    if (rest > 0x1000u || (rest == 0x1000u && (out & 1u) == 1u)) { out += 1u; }
+    return sign | out;
+}
+
+EVO_HD float h2f(uint32_t h) {
+    const uint32_t sign = ((h >> 15) & 1u) << 31;
+    const uint32_t exponent = (h >> 10) & 0x1fu;
+    const uint32_t mantissa = h & 0x3ffu;
+    uint32_t bits;
+    if (exponent == 0u && mantissa == 0u) {
+        bits = sign;
+    } else if (exponent == 0u) {
+        // Normalize a subnormal: the shift puts the leading one at bit 10.
+        const uint32_t shift = static_cast<uint32_t>(__builtin_clz(mantissa)) - 21u;
+        bits = sign | ((113u - shift) << 23) | (((mantissa << shift) & 0x3ffu) << 13);
+    } else if (exponent == 31u && mantissa == 0u) {
+        bits = sign | 0x7f800000u;
+    } else if (exponent == 31u) {
+        bits = sign | 0x7fc00000u | (mantissa << 13);
+    } else {
+        bits = sign | ((exponent + 112u) << 23) | (mantissa << 13);
+    }
+    return bits_f(bits);
+}
+
+}  // namespace evo

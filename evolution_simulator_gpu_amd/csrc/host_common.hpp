@@ -1,0 +1,41 @@
+// Host helpers shared by the library's C++ files.
+#pragma once
+
+#include <cstdint>
+#include <cstring>
+
+#include "../../include/evo_engine.h"
+
+namespace evo {
+
+inline float as_float(uint32_t u) { float f; std::memcpy(&f, &u, 4); return f; }
+inline uint32_t as_uint(float f) { uint32_t u; std::memcpy(&u, &f, 4); return u; }
+
+uint32_t quake_hash(uint64_t id);
+float quake_phase(uint32_t hash);
+float quake_scale(uint32_t hash);
+float ground_height(const evo_world_config& c, float x, float amplitude, float phase);
+
+// IEEE half, round to nearest even (src/creature_kernel.rs:242-274).
+uint32_t f32_to_f16(float v);
+float f16_to_f32(uint32_t h);
+
+// A counter-based random stream: splitmix64 over (seed, key, counter).
+struct Stream {
+    uint64_t state;
+    Stream(uint64_t seed, uint64_t key) : state(mix(seed ^ mix(key + 0x9e3779b97f4a7c15ull))) {}
+    static uint64_t mix(uint64_t z) {
+        z += 0x9e3779b97f4a7c15ull;
+        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+        z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+        return z ^ (z >> 31);
+    }
+    uint64_t next() { state += 0x9e3779b97f4a7c15ull; return mix(state); }
+    // Uniform in [0, 1).
+    float unit() { return static_cast<float>(next() >> 40) * (1.0f / 16777216.0f); }
+    float range(float lo, float hi) { return lo + (hi - lo) * unit(); }
+    // Uniform on [0, n).
+    uint32_t index(uint32_t n) { return n == 0 ? 0 : static_cast<uint32_t>((next() >> 32) % n); }
+};
+
+}  // namespace evo

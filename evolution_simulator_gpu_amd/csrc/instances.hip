@@ -1,0 +1,83 @@
+// The kernel instances of one (steps per second, substeps) pair, chosen by
+// EVO_RATE and EVO_SUB at compile time. The calm flat world (ground only, the
+// world of the plain benchmarks) has its own instances; one any-world
+// instance per body-size class reads the effect bits from a scalar register.
+// Recording kernels run one or a few creatures, so they exist only in the
+// run-time-loop class.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "launch.hpp"
+#include "sim_kernel.hpp"
+
+#ifndef EVO_RATE
+#error "EVO_RATE must be defined"
+#endif
+#ifndef EVO_SUB
+#error "EVO_SUB must be defined"
+#endif
+
+#define EVO_CAT2(a, b, c, d) a##b##c##d
+#define EVO_CAT(a, b, c, d) EVO_CAT2(a, b, c, d)
+#define EVO_PICK EVO_CAT(pick_r, EVO_RATE, _s, EVO_SUB)
+
+namespace evo {
+namespace dev {
+namespace {
+
+constexpr uint32_t kFlat = EVO_GROUND;
+
+template <class C>
+int max_resident_blocks() {
+    static int blocks = -1;
+    if (blocks < 0) {
+        int dev = 0, cus = 0, per_cu = 0;
+        if (hipGetDevice(&dev) != hipSuccess) { return 1; }
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) { return 1; }
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, advance_kernel<C>, C::BLOCK, 0) != hipSuccess) {
+            per_cu = 1;
+        }
+        blocks = std::max(1, cus * std::max(1, per_cu));
+    }
+    return blocks;
+}
+
+template <class C>
+hipError_t launch(const uint32_t* records, const float* muscles, const uint4* heads, evo_result* results,
+                  uint32_t* counter, const evo_params& p, uint32_t flags, float2* frames, uint64_t fpc,
+                  hipStream_t stream, int grid_cap) {
+    const int need = static_cast<int>((p.count + C::BLOCK - 1u) / C::BLOCK);
+    int grid = std::min(need, max_resident_blocks<C>());
+    if (grid_cap > 0) { grid = std::min(grid, grid_cap); }
+    grid = std::max(grid, 1);
+    hipLaunchKernelGGL(clear_words<0>, dim3(1), dim3(64), 0, stream, counter, 1u);
+    hipLaunchKernelGGL(advance_kernel<C>, dim3(grid), dim3(C::BLOCK), 0, stream, records, muscles, heads, results,
+                       counter, p, flags, frames, fpc);
+    return hipGetLastError();
+}
+
+template <uint32_t WORLD>
+Launcher pick_size(const evo_run& run) {
+    constexpr uint32_t R = EVO_RATE, S = EVO_SUB;
+    const bool known = run.max_nodes != 0u;
+    if (known && run.max_nodes <= static_cast<uint32_t>(kSmallN) && run.max_muscles <= static_cast<uint32_t>(kSmallM)) {
+        return &launch<Cfg<R, S, WORLD, kSmallN, kSmallM, false, 128, true>>;
+    }
+    if (known && run.max_nodes <= static_cast<uint32_t>(kMidN) && run.max_muscles <= static_cast<uint32_t>(kMidM)) {
+        return &launch<Cfg<R, S, WORLD, kMidN, kMidM, false, 128, true>>;
+    }
+    return &launch<Cfg<R, S, WORLD, EVO_MAX_NODES, EVO_MAX_MUSCLES, false, 64, false>>;
+}
+
+}  // namespace
+
+Launcher EVO_PICK(const evo_run& run) {
+    if (run.record) {
+        return &launch<Cfg<EVO_RATE, EVO_SUB, kAnyWorld, EVO_MAX_NODES, EVO_MAX_MUSCLES, true, 64, false>>;
+    }
+    return run.flags == kFlat ? pick_size<kFlat>(run) : pick_size<kAnyWorld>(run);
+}
+
+}  // namespace dev
+}  // namespace evo

@@ -1,0 +1,208 @@
+// World settings -> launch parameters, and the host's copy of the ground.
+//
+// evo_world_params restates src/kernel.rs:82-106 (`world_flags`) and
+// :212-251 (`params`) over the settings of src/config.rs:15-115 that a trial
+// reads. The ground functions restate src/physics.rs:523-714; the start pose
+// of a body (synth.cpp, pack.cpp) puts the body on this ground.
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+
+#include "../../include/evo_engine.h"
+#include "host_common.hpp"
+
+namespace evo {
+
+uint32_t quake_hash(uint64_t id) {
+    // src/physics.rs:621-631
+    uint32_t x = static_cast<uint32_t>(id);
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
+float quake_phase(uint32_t hash) { return static_cast<float>(hash & 0xffffu) * (1.0f / 65536.0f); }
+float quake_scale(uint32_t hash) {
+    return 0.6f + static_cast<float>((hash >> 16) & 0xffffu) * (0.8f / 65536.0f);
+}
+
+// src/physics.rs:538-552
+static void terrain_phase(float x, float amplitude, float phase, float& h, float& s) {
+    static const float waves[2][3] = {{1.1f, 0.65f, 0.0f}, {0.43f, 0.35f, 0.3f}};
+    float height = 0.0f, slope = 0.0f;
+    for (const auto& w : waves) {
+        const float t = x / w[0] + w[2] + phase;
+        const float u = t - std::floor(t);
+        const float q = u * (1.0f - u);
+        height += w[1] * 16.0f * q * q;
+        slope += w[1] * 32.0f * q * (1.0f - 2.0f * u) / w[0];
+    }
+    h = amplitude * height;
+    s = amplitude * slope;
+}
+
+// src/physics.rs:573-596
+static void gaps(float x, float width, float& h, float& s) {
+    h = 0.0f;
+    s = 0.0f;
+    if (width <= 0.0f) { return; }
+    const float spacing = 2.0f + 4.0f * width;
+    const float r = x - std::floor(x / spacing) * spacing;
+    const float center = 0.5f * spacing;
+    const float distance = std::fabs(r - center);
+    const float half = 0.5f * width;
+    const float run = std::fmin(0.15f, half);
+    float factor;
+    if (distance >= half) { factor = 0.0f; } else if (distance <= half - run) { factor = 1.0f; } else { factor = (half - distance) / run; }
+    float fs = 0.0f;
+    if (distance > half - run && distance < half) { fs = r < center ? 1.0f / run : -1.0f / run; }
+    h = -2.0f * factor;
+    s = -2.0f * fs;
+}
+
+// src/physics.rs:612-635
+static void hurdles(float x, float height, float& h, float& s) {
+    h = 0.0f;
+    s = 0.0f;
+    if (height <= 0.0f) { return; }
+    const float spacing = 3.0f;
+    const float r = x - std::floor(x / spacing) * spacing;
+    const float center = 0.5f * spacing;
+    const float distance = std::fabs(r - center);
+    const float half = 0.6f;
+    const float run = 0.2f;
+    float factor;
+    if (distance <= half) { factor = 1.0f; } else if (distance >= half + run) { factor = 0.0f; } else { factor = (half + run - distance) / run; }
+    float fs = 0.0f;
+    if (distance > half && distance < half + run) { fs = r < center ? 1.0f / run : -1.0f / run; }
+    h = height * factor;
+    s = height * fs;
+}
+
+// src/physics.rs:660-676 — ground height under x for one creature.
+float ground_height(const evo_world_config& c, float x, float amplitude, float phase) {
+    if (!c.ground) { return -INFINITY; }
+    float h, s, gh, gs, hh, hs;
+    terrain_phase(x, amplitude, phase, h, s);
+    h += c.slope * x;
+    gaps(x, c.gaps, gh, gs);
+    hurdles(x, c.hurdles, hh, hs);
+    return h + gh + hh;
+}
+
+float terrain_amplitude_of(const evo_world_config& c) { return c.terrain; }
+
+}  // namespace evo
+
+extern "C" {
+
+void evo_world_default(evo_world_config* c) {
+    // src/config.rs:118-162 (the fields a trial reads) and
+    // src/physics.rs:10-101 (standard fidelity).
+    std::memset(c, 0, sizeof(*c));
+    c->duration = 20.0f;
+    c->gravity = 9.8f;
+    c->air_retention = 1.0f;
+    c->ground_friction = 1.5f;
+    c->ground = 1;
+    c->muscle_energy = 1.0f;
+    c->muscle_recovery = 1.0f;
+    c->rate = 60;
+    c->substeps = 16;
+    c->screen_seconds = 0.0f;
+    c->screen_bar = -INFINITY;
+    c->screen_bar_young = -INFINITY;
+    c->screen_bar_reshaped = -INFINITY;
+    for (int r = 0; r < 2; r++) {
+        evo_rung& g = r == 0 ? c->r1 : c->r2;
+        std::memset(&g, 0, sizeof(g));
+        g.bias = -INFINITY;
+    }
+}
+
+uint32_t evo_world_flags(const evo_world_config* c) {
+    // src/kernel.rs:82-106
+    const bool ground = c->ground != 0;
+    uint32_t flags = 0;
+    auto set = [&](uint32_t bit, bool on) { if (on) { flags |= bit; } };
+    const float air_step = std::pow(c->air_retention, 60.0f / static_cast<float>(c->rate));
+    set(EVO_GROUND, ground);
+    set(EVO_TERRAIN, ground && (c->terrain != 0.0f || c->quake > 0.0f));
+    set(EVO_SLOPE, ground && c->slope != 0.0f);
+    set(EVO_GAPS, ground && c->gaps > 0.0f);
+    set(EVO_HURDLES, ground && c->hurdles > 0.0f);
+    set(EVO_QUAKE, ground && c->quake > 0.0f);
+    set(EVO_MUD, ground && c->mud > 0.0f);
+    set(EVO_WATER, c->water > 0.0f);
+    set(EVO_ICE, ground && c->patches > 0.0f);
+    set(EVO_WIND, c->wind != 0.0f);
+    set(EVO_AIR, air_step != 1.0f);
+    set(EVO_BRAMBLES, ground && c->brambles > 0.0f);
+    return flags;
+}
+
+int evo_world_params(const evo_world_config* c, uint32_t base, uint32_t count, uint32_t stride, evo_params* p,
+                     evo_run* run) {
+    if (c->rate == 0 || c->substeps == 0 || !(c->duration > 0.0f) || c->muscle_energy <= 0.0f) { return EVO_ERR_INVALID; }
+    // src/kernel.rs:212-251, src/config.rs:636-638, src/physics.rs:412-414
+    const bool ground = c->ground != 0;
+    const float rate = static_cast<float>(c->rate);
+    const float air = std::pow(c->air_retention, 60.0f / rate);
+    const uint32_t settle = (200u * c->rate + 119u) / 120u;
+    std::memset(p, 0, sizeof(*p));
+    p->count = count;
+    p->base = base;
+    p->steps = static_cast<uint32_t>(std::lround(c->duration * rate));
+    if (p->steps == 0) { return EVO_ERR_INVALID; }
+    if (c->screen_seconds > 0.0f) {
+        const long tick = static_cast<long>(settle) +
+                          std::max(1L, static_cast<long>(std::lround(c->screen_seconds * rate))) - 1L;
+        p->screen_step = static_cast<uint32_t>(tick - static_cast<long>(settle));
+        p->screen_bar = c->screen_bar;
+        p->screen_bar_young = c->screen_bar_young;
+        p->screen_bar_reshaped = c->screen_bar_reshaped;
+    } else {
+        p->screen_step = 0xffffffffu;
+        p->screen_bar = -INFINITY;
+        p->screen_bar_young = -INFINITY;
+        p->screen_bar_reshaped = -INFINITY;
+    }
+    p->stride = stride;
+    p->gravity = c->gravity;
+    p->air = air;
+    p->friction = c->ground_friction;
+    p->terrain = c->terrain;
+    p->muscle_energy = c->muscle_energy;
+    p->muscle_recovery = c->muscle_recovery;
+    p->slope = ground ? c->slope : 0.0f;
+    p->wind = c->wind;
+    p->mud = ground ? c->mud : 0.0f;
+    p->gaps = ground ? c->gaps : 0.0f;
+    p->hurdles = ground ? c->hurdles : 0.0f;
+    p->quake = ground ? c->quake : 0.0f;
+    p->water = c->water;
+    p->patches = ground ? c->patches : 0.0f;
+    p->air_sub = std::pow(air, 1.0f / static_cast<float>(c->substeps));
+    p->inv_muscle_energy = 1.0f / c->muscle_energy;
+    p->brambles = ground ? c->brambles : 0.0f;
+    p->r1 = c->r1;
+    p->r2 = c->r2;
+    run->flags = evo_world_flags(c);
+    run->rate = c->rate;
+    run->substeps = c->substeps;
+    run->record = 0;
+    return EVO_OK;
+}
+
+uint32_t evo_settle_frames(uint32_t rate) { return (200u * rate + 119u) / 120u; }
+
+uint32_t evo_quake_hash(uint64_t id) { return evo::quake_hash(id); }
+
+float evo_ground_height(const evo_world_config* c, float x, float amplitude, float phase) {
+    return evo::ground_height(*c, x, amplitude, phase);
+}
+
+}  // extern "C"

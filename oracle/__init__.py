@@ -1,0 +1,10 @@
+"""ORACLE — test infrastructure only.
+
+CPU restatement of the reference's creature trial (shaders/creature.cu) in C,
+built as oracle/_build/liboracle.so by oracle/Makefile. Only tests/,
+__graft_entry__.smoke() and bench.py's CPU-baseline leg may use it, as the
+checker. The product never imports it.
+"""
+from .oracle import build, load, run, run_f32, run_f64
+
+__all__ = ["build", "load", "run", "run_f32", "run_f64"]

@@ -1,0 +1,71 @@
+"""ORACLE — test infrastructure only (see oracle/__init__.py).
+
+ctypes access to liboracle.so: runs a packed batch (the layouts of
+include/evo_types.h) on host threads in FP32 (the bit-exact checker of the
+HIP kernel) or FP64 (the same physics in double precision).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "_build" / "liboracle.so"
+_lib = None
+
+
+def build() -> Path:
+    subprocess.run(["make", "-C", str(HERE), "-s"], check=True)
+    return LIB
+
+
+def load() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            build()
+        lib = C.CDLL(str(LIB))
+        for name in ("oracle_advance_f32", "oracle_advance_f64"):
+            fn = getattr(lib, name)
+            fn.argtypes = [C.c_void_p] * 7 + [C.c_int, C.POINTER(C.c_long)]
+            fn.restype = C.c_int
+        lib.oracle_cos_0_pi_n.argtypes = [C.c_void_p, C.c_void_p, C.c_long]
+        lib.oracle_atan2_n.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_long]
+        lib.oracle_f2h_n.argtypes = [C.c_void_p, C.c_void_p, C.c_long]
+        lib.oracle_h2f_n.argtypes = [C.c_void_p, C.c_void_p, C.c_long]
+        _lib = lib
+    return _lib
+
+
+def run(batch, params, run_, precision: str = "f32", threads: int = 1, frames: bool = False):
+    """Simulates creatures [params.base, params.base + params.count) of
+    `batch`. Returns (results as the 76-byte record array, frames or None,
+    total steps simulated)."""
+    from evolution_simulator_gpu_amd import _native as N
+
+    lib = load()
+    results = np.zeros(batch.count, dtype=N.RESULT_DTYPE)
+    fr = None
+    if frames:
+        fpc = N.lib().evo_settle_frames(run_.rate) + params.steps + 1
+        fr = np.zeros(params.count * fpc * params.stride * 2, dtype=np.float32 if precision == "f32" else np.float64)
+        run_.record = 1
+    steps = C.c_long(0)
+    fn = lib.oracle_advance_f32 if precision == "f32" else lib.oracle_advance_f64
+    rc = fn(batch.records.ctypes.data, batch.muscles.ctypes.data, batch.heads.ctypes.data, C.addressof(params),
+            C.addressof(run_), results.ctypes.data, fr.ctypes.data if fr is not None else None, threads,
+            C.byref(steps))
+    if rc != 0:
+        raise ValueError(f"oracle cannot run creature {-rc - 1}")
+    return results, fr, steps.value
+
+
+def run_f32(batch, params, run_, threads: int = 1, frames: bool = False):
+    return run(batch, params, run_, "f32", threads, frames)
+
+
+def run_f64(batch, params, run_, threads: int = 1, frames: bool = False):
+    return run(batch, params, run_, "f64", threads, frames)

@@ -1,0 +1,124 @@
+/* ORACLE — test infrastructure only. Nothing in the product may include or
+ * link this file; tests, smoke() and the CPU-baseline leg of bench.py use it
+ * as the checker.
+ *
+ * Single-precision elementary functions of the oracle, written from published
+ * methods (no vendor math library is reproduced):
+ *   - orc_cos_0_pi: cos on [0, pi] as -sin(x - pi/2), with the Cody-Waite
+ *     two-constant reduction (Cody & Waite, "Software Manual for the
+ *     Elementary Functions", 1980) and a least-squares polynomial on Chebyshev
+ *     nodes for sin(y) = y + y^3 P(y^2), |y| <= pi/2 + 0.02.
+ *   - orc_atan2: octant reduction to a = min/max in [0, 1], atan(a) = a Q(a^2)
+ *     with a least-squares polynomial on Chebyshev nodes, then the quadrant
+ *     fix-ups of atan2 (IEEE 754-2008 9.2.1 sign rules).
+ *   - orc_f2h / orc_h2f: IEEE binary16 conversions, round to nearest even,
+ *     with the semantics of src/creature_kernel.rs:221-274.
+ * The coefficients are the float32 roundings of the fits; their error bounds
+ * are pinned against float64 numpy in tests/test_oracle.py. The HIP kernel
+ * evaluates the same polynomials in the same order, so the two agree bit for
+ * bit.
+ */
+#ifndef ORACLE_MATH_H
+#define ORACLE_MATH_H
+
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+
+static inline float orc_as_float(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
+static inline uint32_t orc_as_uint(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
+
+/* cos(x) for x in [0, pi] (slightly beyond is fine). */
+static inline float orc_cos_0_pi(float x) {
+    const float c1 = 0x1.921fb6p+0f;   /* float(pi/2) */
+    const float c2 = -0x1.777a5cp-25f; /* pi/2 - c1 */
+    const float y = (x - c1) - c2;
+    const float z = y * y;
+    float p = -0x1.9914fcp-26f;
+    p = fmaf(p, z, 0x1.715fc4p-19f);
+    p = fmaf(p, z, -0x1.a01744p-13f);
+    p = fmaf(p, z, 0x1.11110ap-7f);
+    p = fmaf(p, z, -0x1.555556p-3f);
+    const float s = fmaf(y * z, p, y);
+    return -s;
+}
+
+/* atan(a) for a in [0, 1]. */
+static inline float orc_atan_unit(float a) {
+    const float z = a * a;
+    float q = -0x1.8d8782p-10f;
+    q = fmaf(q, z, 0x1.3ac9dcp-7f);
+    q = fmaf(q, z, -0x1.d44950p-6f);
+    q = fmaf(q, z, 0x1.c3bef6p-5f);
+    q = fmaf(q, z, -0x1.50cd90p-4f);
+    q = fmaf(q, z, 0x1.bdbc88p-4f);
+    q = fmaf(q, z, -0x1.23cec0p-3f);
+    q = fmaf(q, z, 0x1.9987eap-3f);
+    q = fmaf(q, z, -0x1.5554f8p-2f);
+    q = fmaf(q, z, 1.0f);
+    return a * q;
+}
+
+static inline float orc_atan2(float y, float x) {
+    const float ax = fabsf(x), ay = fabsf(y);
+    const float mx = fmaxf(ax, ay), mn = fminf(ax, ay);
+    float a = mn / mx;
+    if (mx == 0.0f) { a = 0.0f; }
+    if (mx == INFINITY && mn == INFINITY) { a = 1.0f; }
+    float t = orc_atan_unit(a);
+    if (ay > ax) { t = 0x1.921fb6p+0f - t; }
+    if (signbit(x)) { t = 0x1.921fb6p+1f - t; }
+    if (x != x || y != y) { t = x + y; }
+    return copysignf(t, y);
+}
+
+/* Single to IEEE half, round to nearest even (src/creature_kernel.rs:242-274). */
+static inline uint32_t orc_f2h(float v) {
+    const uint32_t bits = orc_as_uint(v);
+    const uint32_t sign = (bits >> 16) & 0x8000u;
+    const int32_t exponent = (int32_t)((bits >> 23) & 0xffu);
+    const uint32_t mantissa = bits & 0x7fffffu;
+    if (exponent == 0xff) { return sign | 0x7c00u | (mantissa != 0u ? 0x200u : 0u); }
+    const int32_t e = exponent - 127 + 15;
+    if (e >= 31) { return sign | 0x7c00u; }
+    if (e <= 0) {
+        if (e < -10) { return sign; }
+        const uint32_t m = mantissa | 0x800000u;
+        const uint32_t shift = (uint32_t)(14 - e);
+        const uint32_t half = 1u << (shift - 1u);
+        const uint32_t rest = m & ((1u << shift) - 1u);
+        uint32_t out = m >> shift;
+        if (rest > half || (rest == half && (out & 1u) == 1u)) { out += 1u; }
+        return sign | out;
+    }
+    const uint32_t rest = mantissa & 0x1fffu;
+    uint32_t out = ((uint32_t)e << 10) | (mantissa >> 13);
+    if (rest > 0x1000u || (rest == 0x1000u && (out & 1u) == 1u)) { out += 1u; }
+    return sign | out;
+}
+
+/* IEEE half to single (src/creature_kernel.rs:221-240). */
+static inline float orc_h2f(uint32_t h) {
+    const uint32_t sign = ((h >> 15) & 1u) << 31;
+    const uint32_t exponent = (h >> 10) & 0x1fu;
+    const uint32_t mantissa = h & 0x3ffu;
+    uint32_t bits;
+    if (exponent == 0u && mantissa == 0u) {
+        bits = sign;
+    } else if (exponent == 0u) {
+        uint32_t shift = 0u;
+        uint32_t m = mantissa;
+        while ((m & 0x400u) == 0u) { m <<= 1; shift++; }
+        /* leading_zeros(mantissa) - 21 == shift for a 10-bit mantissa */
+        bits = sign | ((113u - shift) << 23) | ((m & 0x3ffu) << 13);
+    } else if (exponent == 31u && mantissa == 0u) {
+        bits = sign | 0x7f800000u;
+    } else if (exponent == 31u) {
+        bits = sign | 0x7fc00000u | (mantissa << 13);
+    } else {
+        bits = sign | ((exponent + 112u) << 23) | (mantissa << 13);
+    }
+    return orc_as_float(bits);
+}
+
+#endif /* ORACLE_MATH_H */

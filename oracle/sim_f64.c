@@ -1,0 +1,23 @@
+/* ORACLE — test infrastructure only (see oracle_math.h).
+ * The FP64 build of sim_impl.h: the same physics in double precision, with
+ * libm cos and atan2, for the tolerance checks of the FP32 paths. */
+#include <math.h>
+#include <stddef.h>
+#include "oracle_math.h"
+#define R double
+#define FN(name) f64_##name
+#define FMA fma
+#define SQRT sqrt
+#define FLOOR floor
+#define ABS fabs
+#define MIN fmin
+#define MAX fmax
+#define COPYSIGN copysign
+#define COS_0_PI cos
+#define ATAN2 atan2
+#define LIT(x) x
+#define SIM_FRAME_T double
+#define SIM_FRAME_BITS(p, lo, hi) do { (p)[0] = (double)(lo); (p)[1] = (double)(hi); } while (0)
+#include "sim_impl.h"
+#include "oracle_api.h"
+ORACLE_DEFINE_RUNNER(f64, double)

@@ -1,0 +1,874 @@
+/* ORACLE — test infrastructure only (see oracle_math.h).
+ *
+ * CPU restatement of the creature trial of shaders/creature.cu, one creature
+ * at a time, written from the source text. It is included twice: by
+ * sim_f32.c with R = float (the bit-exact checker of the HIP kernel, which
+ * evaluates the same expressions in the same order) and by sim_f64.c with
+ * R = double (the FP64 reference of the same physics, with libm cos/atan2).
+ *
+ * Arithmetic specification shared with the HIP kernel (the places where the
+ * reference leaves rounding to its compiler, which contracts a*b+c and uses
+ * approximate division, are fixed here once):
+ *   - contraction is off (-ffp-contract=off); every fused multiply-add is an
+ *     explicit FMA(a, b, c) = a*b + c rounded once;
+ *   - sqrt and division are IEEE correctly rounded;
+ *   - several quotients by one denominator are one reciprocal and products
+ *     (muscle direction, joint gradients, ground normal);
+ *   - cos and atan2 are the routines of oracle_math.h (FP32) or libm (FP64).
+ *
+ * The including file defines R, FN(name), FMA, SQRT, FLOOR, ABS, MIN, MAX,
+ * COPYSIGN, COS_0_PI, ATAN2, ISFINITE, LIT(x) and SIM_STORE (how a real value
+ * is written into an output float).
+ */
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include "../include/evo_types.h"
+#include "oracle_math.h"
+
+#define MAXN_ EVO_MAX_NODES
+#define MAXM_ EVO_MAX_MUSCLES
+
+/* The world a batch runs in, in R (shaders/creature.cu:75-103). */
+typedef struct {
+    uint32_t flags, steps, screen_step, stride, record;
+    uint32_t rate_u, substeps_u;
+    R rate, dt, h, inv_h;
+    R gravity, friction, terrain, muscle_recovery, slope, wind, mud, gaps, hurdles, quake;
+    R screen_bar, screen_bar_young, screen_bar_reshaped, water, patches, air_sub, inv_muscle_energy, brambles;
+    R rec_h;        /* MUSCLE_RECOVERY * muscle_recovery * H, once per world */
+    R damp_share;   /* min(H / 0.1 s, 1) */
+    evo_rung r1, r2;
+    uint32_t settle, sample, head_sample;
+} FN(World);
+
+/* One creature's constants and state (shaders/creature.cu:255-286). */
+typedef struct {
+    uint32_t nodes, bones, muscles;
+    R inv_mass, amp, qphase;
+    R nm[MAXN_ + 1], nw[MAXN_ + 1], nr[MAXN_ + 1], nf[MAXN_ + 1];
+    uint32_t feet;
+    uint32_t bi0[MAXN_];
+    R blen[MAXN_], bw0[MAXN_], bw1[MAXN_], bk[MAXN_], blim[MAXN_], bs0[MAXN_], bs1[MAXN_];
+    uint32_t jtopo[MAXN_];
+    R jlo[MAXN_], jhi[MAXN_], jmid[MAXN_], jwt[MAXN_], jwf[MAXN_], jwh[MAXN_];
+    R mstore[MAXM_], mdemand[MAXM_], moffset[MAXM_], mpull[MAXM_];
+    uint32_t mnodes[MAXM_];
+    R maa[MAXM_], mab[MAXM_], mhill[MAXM_], mcap[MAXM_], minvcap[MAXM_], mtk[MAXM_], mslack[MAXM_];
+    R normal_force[MAXN_], friction_force[MAXN_];
+    R px[MAXN_], py[MAXN_], vx[MAXN_], vy[MAXN_], fx[MAXN_], fy[MAXN_];
+} FN(Body);
+
+/* The trial so far (shaders/creature.cu:783-826, Result + Tally + Lane). */
+typedef struct {
+    R fitness, ground_contact, vmin, vmax, previous_center_y, vertical_extremum, vertical_trend, gait_turns;
+    R height_sum, fall_time, head_shake_out, screen_x, screened;
+    uint32_t contact_lo, contact_hi, lift_lo, lift_hi, ground_lo, ground_hi;
+    uint32_t contact_bits, lift_bits, ground_bits;
+    R head_shake, head_at_x, head_at_y, head_vel_x, head_vel_y;
+    uint32_t rung_x_bits, rung_speed, rung_early, rung_late, rung_bits;
+    uint32_t flags, step, mus_off;
+    R rung_x;
+    float period_f;
+    int limp, scoring;
+    evo_result kept;
+} FN(Trial);
+
+static inline R FN(clampr)(R x, R lo, R hi) { return MIN(MAX(x, lo), hi); }
+/* shaders/creature.cu:133 — a plus whole turns, in [-pi, pi). */
+static inline R FN(wrapr)(R a) {
+    const R tau = LIT(6.28318530718);
+    return FMA(-tau, FLOOR((a + LIT(3.14159265359)) * (LIT(1.0) / tau)), a);
+}
+static inline R FN(fracr)(R x) { return x - FLOOR(x); }
+
+/* shaders/creature.cu:153-159 — the rhythm, 0 at the longest, 1 at the
+ * shortest, rising over `duty` and falling over the rest as half cosines. */
+static inline R FN(rhythm)(R t, R inv_period, R phase, R offset, R duty, R inv_duty, R inv_complement) {
+    const R ph = FN(fracr)(FMA(t, inv_period, phase) + offset);
+    const R pi = LIT(3.14159265359);
+    if (ph < duty) {
+        return FMA(LIT(-0.5), COS_0_PI(pi * ph * inv_duty), LIT(0.5));
+    }
+    return FMA(LIT(0.5), COS_0_PI(pi * (ph - duty) * inv_complement), LIT(0.5));
+}
+
+/* shaders/creature.cu:166-226 — ground height and slope under x. */
+static void FN(ground)(const FN(World)* w, R x, R amp, R qphase, R* out_h, R* out_s) {
+    R height = LIT(0.0), slope = LIT(0.0);
+    if (w->flags & EVO_TERRAIN) {
+        const R t0 = FMA(x, LIT(1.0) / LIT(1.1), qphase);
+        const R u0 = FN(fracr)(t0);
+        const R w0 = u0 * (LIT(1.0) - u0);
+        const R t1 = FMA(x, LIT(1.0) / LIT(0.43), LIT(0.3)) + qphase;
+        const R u1 = FN(fracr)(t1);
+        const R w1 = u1 * (LIT(1.0) - u1);
+        height = amp * FMA(LIT(0.65) * LIT(16.0) * w0, w0, LIT(0.35) * LIT(16.0) * w1 * w1);
+        slope = amp * FMA(LIT(0.65) * LIT(32.0) * w0 * (LIT(1.0) - LIT(2.0) * u0), LIT(1.0) / LIT(1.1),
+                          LIT(0.35) * LIT(32.0) * w1 * (LIT(1.0) - LIT(2.0) * u1) * (LIT(1.0) / LIT(0.43)));
+    }
+    if (w->flags & EVO_SLOPE) {
+        height = FMA(w->slope, x, height);
+        slope += w->slope;
+    }
+    if (w->flags & EVO_GAPS) {
+        const R spacing = FMA(LIT(4.0), w->gaps, LIT(2.0));
+        const R center = spacing * LIT(0.5);
+        const R r = x - FLOOR(x / spacing) * spacing;
+        const R distance = ABS(r - center);
+        const R half_w = LIT(0.5) * w->gaps;
+        const R run = MAX(MIN(LIT(0.15), half_w), LIT(1e-6));
+        R factor = FN(clampr)((half_w - distance) / run, LIT(0.0), LIT(1.0));
+        if (distance <= half_w - run) { factor = LIT(1.0); } else if (distance >= half_w) { factor = LIT(0.0); }
+        height = FMA(-LIT(2.0), factor, height);
+        if (distance > half_w - run && distance < half_w) {
+            slope -= LIT(2.0) * ((r < center ? LIT(1.0) : LIT(-1.0)) / run);
+        }
+    }
+    if (w->flags & EVO_HURDLES) {
+        const R spacing = LIT(3.0);
+        const R center = spacing * LIT(0.5);
+        const R r = x - FLOOR(x / spacing) * spacing;
+        const R distance = ABS(r - center);
+        const R half_w = LIT(0.5) * LIT(1.2);
+        const R run = LIT(0.2);
+        R factor = FN(clampr)((half_w + run - distance) / run, LIT(0.0), LIT(1.0));
+        if (distance <= half_w) { factor = LIT(1.0); } else if (distance >= half_w + run) { factor = LIT(0.0); }
+        height = FMA(w->hurdles, factor, height);
+        if (distance > half_w && distance < half_w + run) {
+            slope += w->hurdles * ((r < center ? LIT(1.0) : LIT(-1.0)) / run);
+        }
+    }
+    *out_h = height;
+    *out_s = slope;
+}
+
+/* shaders/creature.cu:230-235 — ice at x, 1 on a patch, smooth edges. */
+static inline R FN(ice_at)(R x) {
+    const R w = FN(fracr)(x * (LIT(1.0) / LIT(6.0)));
+    const R t = ABS(w - LIT(0.5)) * LIT(2.0);
+    const R s = FN(clampr)((LIT(0.7) - t) * LIT(2.5), LIT(0.0), LIT(1.0));
+    return s * s * FMA(LIT(-2.0), s, LIT(3.0));
+}
+
+/* shaders/creature.cu:320-356 — a joint's angle and gradients. */
+typedef struct {
+    uint32_t tip, far_, hub;
+    R gtx, gty, gfx, gfy, ghx, ghy, angle, lo, hi, wt, wf, wh;
+} FN(Joint);
+
+static inline void FN(joint)(const FN(Body)* b, uint32_t j, int with_angle, FN(Joint)* k) {
+    const uint32_t topo = b->jtopo[j];
+    k->tip = j + 1u;
+    k->hub = topo & 31u;
+    const uint32_t ptip = (topo >> 5u) & 31u, ppiv = (topo >> 10u) & 31u;
+    k->far_ = (topo >> 15u) & 31u;
+    const int at_head = ((topo >> 20u) & 1u) != 0u;
+    k->lo = b->jlo[j];
+    k->hi = b->jhi[j];
+    k->wt = b->jwt[j];
+    k->wf = b->jwf[j];
+    k->wh = b->jwh[j];
+    const R ux = b->px[ptip] - b->px[ppiv], uy = b->py[ptip] - b->py[ppiv];
+    const R vx = b->px[k->tip] - b->px[k->hub], vy = b->py[k->tip] - b->py[k->hub];
+    const R uu = MAX(FMA(ux, ux, uy * uy), LIT(1e-12));
+    const R vv = MAX(FMA(vx, vx, vy * vy), LIT(1e-12));
+    k->angle = LIT(0.0);
+    if (with_angle) {
+        const R turn = ATAN2(FMA(ux, vy, -(uy * vx)), FMA(ux, vx, uy * vy));
+        const R mid = b->jmid[j];
+        k->angle = mid + FN(wrapr)(turn - mid);
+    }
+    const R ivv = LIT(1.0) / vv, iuu = LIT(1.0) / uu;
+    k->gtx = -vy * ivv;
+    k->gty = vx * ivv;
+    const R gux = -uy * iuu, guy = ux * iuu;
+    k->gfx = at_head ? -gux : gux;
+    k->gfy = at_head ? -guy : guy;
+    k->ghx = -(k->gtx + k->gfx);
+    k->ghy = -(k->gty + k->gfy);
+}
+
+/* shaders/creature.cu:358-380 */
+static inline R FN(joint_weight)(const FN(Joint)* k) {
+    const R nt = FMA(k->gtx, k->gtx, k->gty * k->gty);
+    const R nf = FMA(k->gfx, k->gfx, k->gfy * k->gfy);
+    const R nh = FMA(k->ghx, k->ghx, k->ghy * k->ghy);
+    return FMA(k->wt, nt, FMA(k->wf, nf, k->wh * nh));
+}
+static inline void FN(joint_push)(R* xs, R* ys, const FN(Joint)* k, R lambda) {
+    const R st = k->wt * lambda, sf = k->wf * lambda, sh = k->wh * lambda;
+    xs[k->tip] = FMA(st, k->gtx, xs[k->tip]);
+    ys[k->tip] = FMA(st, k->gty, ys[k->tip]);
+    xs[k->far_] = FMA(sf, k->gfx, xs[k->far_]);
+    ys[k->far_] = FMA(sf, k->gfy, ys[k->far_]);
+    xs[k->hub] = FMA(sh, k->ghx, xs[k->hub]);
+    ys[k->hub] = FMA(sh, k->ghy, ys[k->hub]);
+}
+
+/* shaders/creature.cu:442-451 — each muscle's demand for the step. */
+static void FN(muscle_demands)(const FN(World)* w, FN(Body)* b, const float* mus, R t, int limp) {
+    for (uint32_t k = 0u; k < b->muscles; k++) {
+        const float* m = mus + k * EVO_MUSCLE_WORDS;
+        const R amplitude = m[8], inv_period = m[9], phase = m[10], duty = m[11];
+        const R inv_duty = m[12], inv_complement = m[13], stiffness = m[14];
+        const R offset = b->moffset[k];
+        const R before = FN(rhythm)(t, inv_period, phase, offset, duty, inv_duty, inv_complement);
+        const R after = FN(rhythm)(t + w->dt, inv_period, phase, offset, duty, inv_duty, inv_complement);
+        const R shortening = amplitude * (after - before) * w->rate;
+        b->mdemand[k] = limp ? LIT(0.0) : MAX(shortening * stiffness * LIT(0.25), LIT(0.0));
+    }
+}
+
+/* shaders/creature.cu:455-508 — the muscles' pull into the force table and
+ * the energy stores. */
+static void FN(muscle_forces)(const FN(World)* w, FN(Body)* b) {
+    for (uint32_t i = 0u; i < b->nodes; i++) { b->fx[i] = LIT(0.0); b->fy[i] = LIT(0.0); }
+    for (uint32_t k = 0u; k < b->muscles; k++) {
+        const uint32_t nodes = b->mnodes[k];
+        const uint32_t a0 = nodes & 255u, a1 = (nodes >> 8u) & 255u, b0 = (nodes >> 16u) & 255u, b1 = nodes >> 24u;
+        const R aa = b->maa[k], ab = b->mab[k];
+        const R ax = FMA(b->px[a1] - b->px[a0], aa, b->px[a0]);
+        const R ay = FMA(b->py[a1] - b->py[a0], aa, b->py[a0]);
+        const R avx = FMA(b->vx[a1] - b->vx[a0], aa, b->vx[a0]);
+        const R avy = FMA(b->vy[a1] - b->vy[a0], aa, b->vy[a0]);
+        const R bx = FMA(b->px[b1] - b->px[b0], ab, b->px[b0]);
+        const R by = FMA(b->py[b1] - b->py[b0], ab, b->py[b0]);
+        const R bvx = FMA(b->vx[b1] - b->vx[b0], ab, b->vx[b0]);
+        const R bvy = FMA(b->vy[b1] - b->vy[b0], ab, b->vy[b0]);
+        const R dx = bx - ax, dy = by - ay;
+        const R len = MAX(SQRT(FMA(dx, dx, dy * dy)), LIT(1e-6));
+        const R inv_len = LIT(1.0) / len;
+        const R ex = dx * inv_len, ey = dy * inv_len;
+        const R lengthening = FMA(bvx - avx, ex, (bvy - avy) * ey);
+        const R store = b->mstore[k];
+        R drive = b->mdemand[k] * store;
+        if (b->mhill[k] > LIT(0.0)) {
+            drive *= FN(clampr)(FMA(lengthening, b->mhill[k], LIT(1.0)), LIT(0.0), LIT(1.0));
+        }
+        const R cap = b->mcap[k];
+        const R active = FN(clampr)(FMA(lengthening, LIT(0.15), drive), -cap, cap);
+        const R work = MIN(drive, cap) * MAX(-lengthening, LIT(0.0)) * w->h;
+        b->mstore[k] = FN(clampr)(FMA(w->rec_h, LIT(1.0) - store, store - work * b->minvcap[k] * w->inv_muscle_energy),
+                                  LIT(0.0), LIT(1.0));
+        const R pull = FMA(b->mtk[k], MAX(len - b->mslack[k], LIT(0.0)), active);
+        if (w->record) { b->mpull[k] = pull; }
+        const R fx = ex * pull, fy = ey * pull;
+        const R ra = LIT(1.0) - aa, rb = LIT(1.0) - ab;
+        b->fx[a0] = FMA(fx, ra, b->fx[a0]);
+        b->fy[a0] = FMA(fy, ra, b->fy[a0]);
+        b->fx[a1] = FMA(fx, aa, b->fx[a1]);
+        b->fy[a1] = FMA(fy, aa, b->fy[a1]);
+        b->fx[b0] = FMA(-fx, rb, b->fx[b0]);
+        b->fy[b0] = FMA(-fy, rb, b->fy[b0]);
+        b->fx[b1] = FMA(-fx, ab, b->fx[b1]);
+        b->fy[b1] = FMA(-fy, ab, b->fy[b1]);
+    }
+}
+
+/* shaders/creature.cu:511-546 — gravity, wind, mud, brambles, buoyancy. */
+static void FN(node_force)(const FN(World)* w, const FN(Body)* b, uint32_t i, R* ofx, R* ofy) {
+    const R m = b->nm[i];
+    R fx = LIT(0.0), fy = -w->gravity * m;
+    if (w->flags & EVO_WIND) { fx = FMA(w->wind, m, fx); }
+    if (w->flags & (EVO_MUD | EVO_BRAMBLES)) {
+        R gh, gs;
+        FN(ground)(w, b->px[i], b->amp, b->qphase, &gh, &gs);
+        const R dry = (b->py[i] - gh) / SQRT(FMA(gs, gs, LIT(1.0))) - b->nr[i];
+        if (w->flags & EVO_MUD) {
+            fx -= m * LIT(2.0) * (FN(clampr)(-dry, LIT(0.0), w->mud) * (LIT(1.0) / LIT(0.1))) * b->vx[i];
+        }
+        if (w->flags & EVO_BRAMBLES) {
+            if (((b->feet >> i) & 1u) == 0u && dry < LIT(0.01)) {
+                fx -= m * w->brambles * b->vx[i];
+            }
+        }
+    }
+    if (w->flags & EVO_WATER) {
+        const R wet = FN(clampr)((w->water - (b->py[i] - b->nr[i])) / (LIT(2.0) * b->nr[i]), LIT(0.0), LIT(1.0));
+        fy = FMA(LIT(0.7) * m * w->gravity, wet, fy);
+    }
+    *ofx = fx;
+    *ofy = fy;
+}
+
+/* shaders/creature.cu:553-593 — air and water drag on each bone. */
+static void FN(bone_drag)(const FN(World)* w, FN(Body)* b) {
+    for (uint32_t j = 0u; j < b->bones; j++) {
+        const uint32_t i0 = b->bi0[j], i1 = j + 1u;
+        const R wx = LIT(0.5) * (b->vx[i0] + b->vx[i1]), wy = LIT(0.5) * (b->vy[i0] + b->vy[i1]);
+        const R speed = SQRT(FMA(wx, wx, wy * wy));
+        const R limit = b->blim[j];
+        const R k_air = MIN(b->bk[j] * speed, limit);
+        R dfx = -wx * k_air, dfy = -wy * k_air;
+        if (w->flags & EVO_WATER) {
+            const R r0 = b->nr[i0], r1 = b->nr[i1];
+            const R width = r0 + r1;
+            const R wet0 = FN(clampr)((w->water - (b->py[i0] - r0)) / (LIT(2.0) * r0), LIT(0.0), LIT(1.0));
+            const R wet1 = FN(clampr)((w->water - (b->py[i1] - r1)) / (LIT(2.0) * r1), LIT(0.0), LIT(1.0));
+            const R wet = LIT(0.5) * (wet0 + wet1);
+            const R ax = (b->px[i1] - b->px[i0]) / b->blen[j], ay = (b->py[i1] - b->py[i0]) / b->blen[j];
+            const R along = FMA(wx, ax, wy * ay);
+            const R lx = ax * along, ly = ay * along;
+            const R sx = wx - lx, sy = wy - ly;
+            const R k_water = MIN(LIT(100.0) * wet * b->blen[j] * width * speed, limit);
+            dfx -= FMA(sx, k_water, lx * k_water * LIT(0.25));
+            dfy -= FMA(sy, k_water, ly * k_water * LIT(0.25));
+        }
+        const R s0 = b->bs0[j], s1 = b->bs1[j];
+        b->vx[i0] = FMA(s0, dfx, b->vx[i0]);
+        b->vy[i0] = FMA(s0, dfy, b->vy[i0]);
+        b->vx[i1] = FMA(s1, dfx, b->vx[i1]);
+        b->vy[i1] = FMA(s1, dfy, b->vy[i1]);
+    }
+}
+
+/* shaders/creature.cu:597-613 — every bone back to its length. */
+static void FN(solve_bones)(FN(Body)* b) {
+    for (uint32_t j = 0u; j < b->bones; j++) {
+        const uint32_t i0 = b->bi0[j], i1 = j + 1u;
+        const R dx = b->px[i1] - b->px[i0], dy = b->py[i1] - b->py[i0];
+        const R len = MAX(SQRT(FMA(dx, dx, dy * dy)), LIT(1e-9));
+        const R w0 = b->bw0[j], w1 = b->bw1[j];
+        const R s = (len - b->blen[j]) / ((w0 + w1) * len);
+        const R s0 = w0 * s, s1 = w1 * s;
+        b->px[i0] = FMA(s0, dx, b->px[i0]);
+        b->py[i0] = FMA(s0, dy, b->py[i0]);
+        b->px[i1] = FMA(-s1, dx, b->px[i1]);
+        b->py[i1] = FMA(-s1, dy, b->py[i1]);
+    }
+}
+
+/* shaders/creature.cu:618-634 — every joint back inside its range. */
+static void FN(solve_joints)(FN(Body)* b) {
+    for (uint32_t j = 1u; j < b->bones; j++) {
+        if (b->jtopo[j] == EVO_NONE) { continue; }
+        FN(Joint) k;
+        FN(joint)(b, j, 1, &k);
+        R error = LIT(0.0);
+        if (k.angle < k.lo) { error = k.angle - k.lo; }
+        if (k.angle > k.hi) { error = k.angle - k.hi; }
+        if (error == LIT(0.0)) { continue; }
+        const R wgt = FN(joint_weight)(&k);
+        if (wgt > LIT(0.0)) { FN(joint_push)(b->px, b->py, &k, -error / wgt); }
+    }
+}
+
+/* shaders/creature.cu:640-679 — node i out of the ground, with friction. */
+static void FN(solve_ground)(const FN(World)* w, FN(Body)* b, uint32_t i) {
+    if (!(w->flags & EVO_GROUND)) { return; }
+    R gh, gs;
+    FN(ground)(w, b->px[i], b->amp, b->qphase, &gh, &gs);
+    const R secant = SQRT(FMA(gs, gs, LIT(1.0)));
+    const R inv_sec = LIT(1.0) / secant;
+    const R nx = -gs * inv_sec, ny = inv_sec;
+    const R dry = FMA(b->py[i] - gh, inv_sec, -b->nr[i]);
+    R depth, mu;
+    if (w->flags & EVO_MUD) {
+        depth = -(dry + w->mud);
+        const R sink = FN(clampr)(-dry, LIT(0.0), w->mud) * (LIT(1.0) / LIT(0.1));
+        mu = b->nf[i] * w->friction * FMA(LIT(2.0), sink, LIT(1.0)) * FMA(LIT(2.0), sink, LIT(1.0));
+    } else {
+        depth = -dry;
+        mu = b->nf[i] * w->friction;
+    }
+    if (depth <= LIT(0.0)) { return; }
+    if (w->flags & EVO_ICE) {
+        mu *= FMA(-w->patches, FN(ice_at)(b->px[i]), LIT(1.0));
+    }
+    b->px[i] = FMA(nx, depth, b->px[i]);
+    b->py[i] = FMA(ny, depth, b->py[i]);
+    const R tx = ny, ty = -nx;
+    const R slide = FMA(b->px[i] - b->fx[i], tx, (b->py[i] - b->fy[i]) * ty);
+    const R budget = mu * depth;
+    const R back = ABS(slide) <= budget ? slide : COPYSIGN(budget, slide);
+    b->px[i] = FMA(-tx, back, b->px[i]);
+    b->py[i] = FMA(-ty, back, b->py[i]);
+    if (w->record) {
+        b->normal_force[i] += b->nm[i] * depth * w->inv_h * w->inv_h;
+        b->friction_force[i] += b->nm[i] * back * w->inv_h * w->inv_h;
+    }
+}
+
+/* shaders/creature.cu:683-702 — joint damping of turning speeds. */
+static void FN(damp_joints)(const FN(World)* w, FN(Body)* b) {
+    for (uint32_t j = 1u; j < b->bones; j++) {
+        if (b->jtopo[j] == EVO_NONE) { continue; }
+        FN(Joint) k;
+        FN(joint)(b, j, 0, &k);
+        R rate = k.gtx * b->vx[k.tip];
+        rate = FMA(k.gty, b->vy[k.tip], rate);
+        rate = FMA(k.gfx, b->vx[k.far_], rate);
+        rate = FMA(k.gfy, b->vy[k.far_], rate);
+        rate = FMA(k.ghx, b->vx[k.hub], rate);
+        rate = FMA(k.ghy, b->vy[k.hub], rate);
+        const R wgt = FN(joint_weight)(&k);
+        if (wgt > LIT(0.0)) { FN(joint_push)(b->vx, b->vy, &k, (-w->damp_share * rate) / wgt); }
+    }
+}
+
+/* shaders/creature.cu:705-742 — one substep. */
+static void FN(substep)(const FN(World)* w, FN(Body)* b) {
+    FN(muscle_forces)(w, b);
+    FN(bone_drag)(w, b);
+    for (uint32_t i = 0u; i < b->nodes; i++) {
+        R fxw, fyw;
+        FN(node_force)(w, b, i, &fxw, &fyw);
+        const R wi = b->nw[i];
+        b->vx[i] = FMA(w->h * (fxw + b->fx[i]), wi, b->vx[i]);
+        b->vy[i] = FMA(w->h * (fyw + b->fy[i]), wi, b->vy[i]);
+        if (w->flags & EVO_AIR) {
+            b->vx[i] *= w->air_sub;
+            b->vy[i] *= w->air_sub;
+        }
+        b->fx[i] = b->px[i];
+        b->fy[i] = b->py[i];
+        b->px[i] = FMA(w->h, b->vx[i], b->px[i]);
+        b->py[i] = FMA(w->h, b->vy[i], b->py[i]);
+    }
+    FN(solve_bones)(b);
+    FN(solve_joints)(b);
+    for (uint32_t i = 0u; i < b->nodes; i++) {
+        FN(solve_ground)(w, b, i);
+        b->vx[i] = (b->px[i] - b->fx[i]) * w->inv_h;
+        b->vy[i] = (b->py[i] - b->fy[i]) * w->inv_h;
+    }
+    FN(damp_joints)(w, b);
+}
+
+/* shaders/creature.cu:746-759 — joints forced past their range by more
+ * than JOINT_BREAK, as bits. */
+static uint64_t FN(broken_joints)(const FN(Body)* b) {
+    uint64_t bits = 0u;
+    for (uint32_t j = 1u; j < b->bones; j++) {
+        if (b->jtopo[j] == EVO_NONE) { continue; }
+        FN(Joint) k;
+        FN(joint)(b, j, 1, &k);
+        if (k.angle < k.lo - LIT(0.5) || k.angle > k.hi + LIT(0.5)) { bits |= (uint64_t)1u << j; }
+    }
+    return bits;
+}
+
+/* shaders/creature.cu:767-779 — frame t of a recording: MAXN positions, an
+ * (energy, pull) pair per muscle, a (normal, friction) pair per node, and the
+ * broken-joint bits in the last slot. */
+static void FN(record_frame)(const FN(World)* w, const FN(Body)* b, SIM_FRAME_T* frames, uint32_t t) {
+    SIM_FRAME_T* f = frames + (size_t)2u * t * w->stride;
+    for (uint32_t i = 0u; i < b->nodes; i++) {
+        f[2u * i] = (SIM_FRAME_T)b->px[i];
+        f[2u * i + 1u] = (SIM_FRAME_T)b->py[i];
+        f[2u * (EVO_MAX_NODES + b->muscles + i)] = (SIM_FRAME_T)b->normal_force[i];
+        f[2u * (EVO_MAX_NODES + b->muscles + i) + 1u] = (SIM_FRAME_T)b->friction_force[i];
+    }
+    for (uint32_t k = 0u; k < b->muscles; k++) {
+        f[2u * (EVO_MAX_NODES + k)] = (SIM_FRAME_T)b->mstore[k];
+        f[2u * (EVO_MAX_NODES + k) + 1u] = (SIM_FRAME_T)b->mpull[k];
+    }
+    const uint64_t broken = FN(broken_joints)(b);
+    SIM_FRAME_BITS(f + 2u * (w->stride - 1u), (uint32_t)broken, (uint32_t)(broken >> 32));
+}
+
+static inline float FN(tof)(R x) { return (float)x; }
+
+/* shaders/creature.cu:845-950 — a creature's constants, start pose and
+ * totals. Returns 0, or -1 for a record that the kernel cannot run. */
+static int FN(begin_creature)(const FN(World)* w, FN(Body)* b, FN(Trial)* tr, uint32_t cidx, const uint32_t* records,
+                              const float* muscles, const uint32_t* heads, SIM_FRAME_T* frames) {
+    const uint32_t* h0 = heads + 8u * (size_t)cidx;
+    const uint32_t* h1 = h0 + 4;
+    memset(b, 0, sizeof(*b));
+    memset(tr, 0, sizeof(*tr));
+    b->nodes = h0[0] & 255u;
+    if (b->nodes < 2u || b->nodes > EVO_MAX_NODES || h0[1] > EVO_MAX_MUSCLES) { return -1; }
+    b->bones = b->nodes - 1u;
+    b->muscles = h0[1];
+    const uint32_t* rec = records + h0[3];
+    tr->mus_off = h1[0];
+    b->inv_mass = orc_as_float(h1[2]);
+    tr->flags = h1[3] >> 16u;
+    tr->period_f = orc_h2f(h1[3] & 0xffffu);
+    if (w->flags & EVO_QUAKE) {
+        b->qphase = (R)(float)(h0[2] & 0xffffu) * (LIT(1.0) / LIT(65536.0));
+        b->amp = FMA(w->quake, FMA((R)(float)((h0[2] >> 16u) & 0xffffu), LIT(0.8) / LIT(65536.0), LIT(0.6)), w->terrain);
+    } else {
+        b->qphase = LIT(0.0);
+        b->amp = w->terrain;
+    }
+    for (uint32_t i = 0u; i < b->nodes; i++) {
+        const uint32_t* r = rec + i * EVO_NODE_WORDS;
+        b->nm[i] = orc_as_float(r[0]);
+        b->nw[i] = LIT(1.0) / b->nm[i];
+        b->nr[i] = orc_as_float(r[1]);
+        b->nf[i] = orc_as_float(r[2]);
+        b->px[i] = orc_as_float(r[3]);
+        b->py[i] = orc_as_float(r[4]);
+        b->feet |= (r[5] & 1u) << i;
+    }
+    const uint32_t* bone_rec = rec + b->nodes * EVO_NODE_WORDS;
+    for (uint32_t j = 0u; j < b->bones; j++) {
+        const uint32_t* r = bone_rec + j * EVO_BONE_WORDS;
+        const uint32_t i0 = r[0], i1 = j + 1u;
+        if (i0 > j) { return -1; }
+        const R length = orc_as_float(r[1]);
+        const R width = b->nr[i0] + b->nr[i1];
+        const R bone_mass = b->nm[i0] + b->nm[i1];
+        b->bi0[j] = i0;
+        b->blen[j] = length;
+        b->bw0[j] = b->nw[i0];
+        b->bw1[j] = b->nw[i1];
+        b->bk[j] = LIT(0.6) * length * width;
+        b->blim[j] = LIT(0.5) * bone_mass * w->inv_h;
+        b->bs0[j] = LIT(0.5) * w->h * b->nw[i0];
+        b->bs1[j] = LIT(0.5) * w->h * b->nw[i1];
+        b->jtopo[j] = EVO_NONE;
+        const uint32_t pj = r[2];
+        if (j > 0u && pj != EVO_NONE) {
+            if (pj >= j) { return -1; }
+            const uint32_t hub = i0, parent_pivot = bone_rec[pj * EVO_BONE_WORDS];
+            const int at_head = hub == parent_pivot;
+            const uint32_t far_ = at_head ? pj + 1u : parent_pivot;
+            b->jtopo[j] = hub | ((pj + 1u) << 5u) | (parent_pivot << 10u) | (far_ << 15u) | ((at_head ? 1u : 0u) << 20u);
+            const R lo = orc_as_float(r[3]), hi = orc_as_float(r[4]);
+            b->jlo[j] = lo;
+            b->jhi[j] = hi;
+            b->jmid[j] = LIT(0.5) * (lo + hi);
+            b->jwt[j] = b->nw[i1];
+            b->jwf[j] = b->nw[far_];
+            b->jwh[j] = b->nw[hub];
+        }
+    }
+    for (uint32_t k = 0u; k < b->muscles; k++) {
+        const float* q = muscles + tr->mus_off + k * EVO_MUSCLE_WORDS;
+        b->mstore[k] = LIT(1.0);
+        b->mnodes[k] = orc_as_uint(q[0]);
+        const uint32_t n = b->mnodes[k];
+        if ((n & 255u) >= b->nodes || ((n >> 8u) & 255u) >= b->nodes || ((n >> 16u) & 255u) >= b->nodes ||
+            (n >> 24u) >= b->nodes) {
+            return -1;
+        }
+        b->maa[k] = q[1];
+        b->mab[k] = q[2];
+        b->mhill[k] = q[3];
+        b->mcap[k] = q[4];
+        b->minvcap[k] = q[5];
+        b->mtk[k] = q[6];
+        b->mslack[k] = q[7];
+    }
+    tr->vmin = LIT(1e20);
+    tr->vmax = LIT(-1e20);
+    tr->head_at_x = b->px[0];
+    tr->head_at_y = b->py[0];
+    tr->limp = 0;
+    tr->step = 0u;
+    tr->scoring = 1;
+    if (w->record) {
+        for (uint32_t t = 0u; t <= w->settle; t++) { FN(record_frame)(w, b, frames, t); }
+    }
+    return 0;
+}
+
+static uint32_t FN(popc)(uint32_t x) { return (uint32_t)__builtin_popcount(x); }
+
+/* The result words as the trial leaves them (shaders/creature.cu:1190-1223). */
+static void FN(pack_result)(const FN(Trial)* tr, evo_result* out) {
+    out->fitness = FN(tof)(tr->fitness);
+    out->ground_contact = FN(tof)(tr->ground_contact);
+    out->vertical_oscillation = FN(tof)(tr->vmin);
+    out->gait_frequency = FN(tof)(tr->vmax);
+    out->previous_center_y = FN(tof)(tr->previous_center_y);
+    out->vertical_extremum = FN(tof)(tr->vertical_extremum);
+    out->vertical_trend = FN(tof)(tr->vertical_trend);
+    out->gait_turns = FN(tof)(tr->gait_turns);
+    out->height_sum = FN(tof)(tr->height_sum);
+    out->contact_lo = orc_as_float(tr->contact_lo);
+    out->contact_hi = orc_as_float(tr->contact_hi);
+    out->lift_lo = orc_as_float(tr->lift_lo);
+    out->lift_hi = orc_as_float(tr->lift_hi);
+    out->ground_lo = orc_as_float(tr->ground_lo);
+    out->ground_hi = orc_as_float(tr->ground_hi);
+    out->fall_time = FN(tof)(tr->fall_time);
+    out->head_shake = FN(tof)(tr->head_shake_out);
+    out->screen_x = FN(tof)(tr->screen_x);
+    out->screened = FN(tof)(tr->screened);
+}
+
+/* shaders/creature.cu:956-1248 — one step. Returns 1 when the creature is
+ * done and its result is written. */
+static int FN(run_step)(const FN(World)* w, FN(Body)* b, FN(Trial)* tr, const float* muscles, evo_result* out,
+                        SIM_FRAME_T* frames) {
+    const uint32_t step = tr->step;
+    const float* mus = muscles + tr->mus_off;
+    const uint32_t flags = tr->flags;
+    const uint32_t rate_u = w->rate_u;
+    const uint32_t rung1 = rate_u - 1u, rung2 = (uint32_t)(LIT(2.5) * (float)rate_u) - 1u;
+    const uint32_t rung3 = 5u * rate_u - 1u, rung4 = 10u * rate_u - 1u;
+    const uint32_t half_s = rate_u / 2u;
+    const R t_now = (R)step * w->dt;
+    if (w->record) {
+        for (uint32_t i = 0u; i < b->nodes; i++) { b->normal_force[i] = LIT(0.0); b->friction_force[i] = LIT(0.0); }
+    }
+    FN(muscle_demands)(w, b, mus, t_now, tr->limp);
+    for (uint32_t s = 0u; s < w->substeps_u; s++) { FN(substep)(w, b); }
+    if (w->record) {
+        const R inv_sub = LIT(1.0) / (R)w->substeps_u;
+        for (uint32_t i = 0u; i < b->nodes; i++) {
+            b->normal_force[i] *= inv_sub;
+            b->friction_force[i] *= inv_sub;
+        }
+    }
+    int failed = 0;
+    R center_y = LIT(0.0), com_x = LIT(0.0), low = LIT(1e20), high = LIT(-1e20);
+    uint32_t touching = 0u, lifted = 0u;
+    for (uint32_t i = 0u; i < b->nodes; i++) {
+        failed = failed || !(ABS(b->px[i]) <= LIT(1e6) && ABS(b->py[i]) <= LIT(1e6));
+        center_y += b->py[i];
+        com_x = FMA(b->nm[i], b->px[i], com_x);
+        low = MIN(low, b->py[i] - b->nr[i]);
+        high = MAX(high, b->py[i] + b->nr[i]);
+        if (w->flags & EVO_GROUND) {
+            R gh, gs;
+            FN(ground)(w, b->px[i], b->amp, b->qphase, &gh, &gs);
+            const R floor_y = FMA(b->nr[i], SQRT(FMA(gs, gs, LIT(1.0))), gh);
+            if (b->py[i] <= floor_y + LIT(0.002)) { touching |= 1u << i; }
+            if (b->py[i] > floor_y + LIT(0.01)) { lifted |= 1u << i; }
+        }
+    }
+    center_y /= (R)b->nodes;
+    com_x *= b->inv_mass;
+    tr->contact_bits |= touching;
+    tr->lift_bits |= tr->contact_bits & lifted;
+    const uint32_t down = touching & ~tr->ground_bits;
+    tr->ground_bits = touching;
+    if (down != 0u && step > 0u) {
+        const R next = t_now + w->dt;
+        for (uint32_t k = 0u; k < b->muscles; k++) {
+            const float* m = mus + k * EVO_MUSCLE_WORDS;
+            const uint32_t sensor = orc_as_uint(m[16]);
+            if (sensor != EVO_NONE && sensor < 32u && ((down >> sensor) & 1u)) {
+                b->moffset[k] = FN(fracr)((R)m[15] - FMA(next, (R)m[9], (R)m[10]));
+            }
+        }
+    }
+    if ((step + 1u) % w->head_sample == 0u) {
+        const R vx = (b->px[0] - tr->head_at_x) * LIT(60.0), vy = (b->py[0] - tr->head_at_y) * LIT(60.0);
+        const R ax = (vx - tr->head_vel_x) * LIT(60.0), ay = (vy - tr->head_vel_y) * LIT(60.0);
+        if (t_now >= LIT(0.1)) {
+            const R accel = SQRT(FMA(ax, ax, ay * ay));
+            tr->head_shake = FMA(accel - tr->head_shake, MIN(LIT(1.0) / (LIT(0.1) * LIT(60.0)), LIT(1.0)), tr->head_shake);
+        }
+        tr->head_at_x = b->px[0];
+        tr->head_at_y = b->py[0];
+        tr->head_vel_x = vx;
+        tr->head_vel_y = vy;
+    }
+    const int broken = FN(broken_joints)(b) != 0u;
+    if (tr->scoring) {
+        tr->contact_lo = tr->contact_bits;
+        tr->lift_lo = tr->lift_bits;
+        tr->ground_lo = tr->ground_bits;
+        tr->head_shake_out = tr->head_shake;
+        const int fell = b->py[0] < b->py[1] || broken || tr->head_shake > LIT(8.0) * LIT(9.8) || failed;
+        int ended = 0;
+        if (fell) {
+            tr->fall_time = t_now + w->dt;
+            tr->fitness = failed ? LIT(-1e20) : com_x;
+            if (step <= w->screen_step) { tr->screen_x = tr->fitness; }
+            ended = 1;
+        }
+        tr->ground_contact += (R)FN(popc)(touching);
+        tr->height_sum += high - low;
+        tr->vmin = MIN(tr->vmin, center_y);
+        tr->vmax = MAX(tr->vmax, center_y);
+        if (step == 0u) {
+            tr->previous_center_y = center_y;
+            tr->vertical_extremum = center_y;
+            tr->vertical_trend = LIT(0.0);
+            tr->gait_turns = LIT(0.0);
+        } else if (step % w->sample == 0u) {
+            const R delta = center_y - tr->previous_center_y;
+            if (tr->vertical_trend == LIT(0.0)) {
+                if (ABS(delta) > LIT(0.0005)) {
+                    tr->vertical_trend = delta > LIT(0.0) ? LIT(1.0) : LIT(-1.0);
+                    tr->vertical_extremum = center_y;
+                }
+            } else if (tr->vertical_trend > LIT(0.0)) {
+                if (center_y > tr->vertical_extremum) {
+                    tr->vertical_extremum = center_y;
+                } else if (tr->vertical_extremum - center_y > LIT(0.005)) {
+                    tr->gait_turns += LIT(1.0);
+                    tr->vertical_trend = LIT(-1.0);
+                    tr->vertical_extremum = center_y;
+                }
+            } else {
+                if (center_y < tr->vertical_extremum) {
+                    tr->vertical_extremum = center_y;
+                } else if (center_y - tr->vertical_extremum > LIT(0.005)) {
+                    tr->gait_turns += LIT(1.0);
+                    tr->vertical_trend = LIT(1.0);
+                    tr->vertical_extremum = center_y;
+                }
+            }
+            tr->previous_center_y = center_y;
+        }
+        if (step + half_s == rung1 || step + half_s == rung2) { tr->rung_x = com_x; }
+        if (step == rung1 || step == rung2) {
+            const int second = step == rung2;
+            const uint32_t sh = second ? 16u : 0u;
+            const uint32_t keep = second ? 0x0000ffffu : 0xffff0000u;
+            const R speed = (com_x - tr->rung_x) * (w->rate / (R)half_s);
+            const R touched = (R)FN(popc)(tr->contact_bits) / (R)b->nodes;
+            R en_mean = LIT(0.0);
+            for (uint32_t k = 0u; k < b->muscles; k++) { en_mean += b->mstore[k]; }
+            en_mean /= (R)(b->muscles > 1u ? b->muscles : 1u);
+            tr->contact_hi = (tr->contact_hi & keep) | (orc_f2h(FN(tof)(com_x)) << sh);
+            tr->ground_hi = (tr->ground_hi & keep) | (orc_f2h(FN(tof)(tr->head_shake)) << sh);
+            tr->rung_speed = (tr->rung_speed & keep) | (orc_f2h(FN(tof)(speed)) << sh);
+            const uint32_t pair = orc_f2h(FN(tof)(touched)) | (orc_f2h(FN(tof)(en_mean)) << 16u);
+            if (second) { tr->rung_late = pair; } else { tr->rung_early = pair; }
+            const uint32_t r = second ? 1u : 0u;
+            const R gait_now = tr->gait_turns * (LIT(0.5) / (t_now + w->dt));
+            const R bandf = gait_now * (LIT(8.0) / LIT(6.0));
+            uint32_t band = bandf >= LIT(7.0) ? 7u : (bandf > LIT(0.0) ? (uint32_t)bandf : 0u);
+            if (bandf != bandf) { band = 0u; }
+            tr->rung_bits |= band << (8u + 3u * r);
+            if (!ended && (flags & (second ? 0xbu : 0x7u)) == 0u) {
+                const evo_rung* rp = second ? &w->r2 : &w->r1;
+                const float fv[6] = {orc_h2f(orc_f2h(FN(tof)(com_x))), orc_h2f(orc_f2h(FN(tof)(speed))),
+                                     orc_h2f(orc_f2h(FN(tof)(touched))), orc_h2f(orc_f2h(FN(tof)(tr->head_shake))),
+                                     orc_h2f(orc_f2h(FN(tof)(en_mean))), tr->period_f};
+                float score = 0.0f;
+                int finite = 1;
+                for (int q = 0; q < 6; q++) {
+                    score = fmaf(rp->w[q], fv[q], score);
+                    finite = finite && isfinite(fv[q]);
+                }
+                if (finite && score < rp->bias && ((rp->off >> band) & 1u) == 0u) {
+                    tr->screened = t_now + w->dt;
+                    tr->screen_x = com_x;
+                    tr->fitness = com_x;
+                    tr->rung_bits |= (r + 1u) << 6u;
+                    ended = 1;
+                }
+            }
+        }
+        if (step == rung3 || step == rung4) {
+            const int second = step == rung4;
+            const uint32_t keep = second ? 0x0000ffffu : 0xffff0000u;
+            tr->lift_hi = (tr->lift_hi & keep) | (orc_f2h(FN(tof)(com_x)) << (second ? 16u : 0u));
+        }
+        if (step == w->screen_step && !ended) {
+            tr->screen_x = com_x;
+            const R bar = (flags & 32u) != 0u ? w->screen_bar_reshaped
+                        : (flags & 16u) != 0u ? w->screen_bar_young : w->screen_bar;
+            if (com_x < bar && (flags & 1u) == 0u) {
+                tr->screened = t_now + w->dt;
+                tr->fitness = com_x;
+                ended = 1;
+            }
+        }
+        int last = 0;
+        if (ended) {
+            tr->vmin = MAX(tr->vmax - tr->vmin, LIT(0.0));
+            tr->vmax = tr->gait_turns * LIT(0.5) / (t_now + w->dt);
+        } else if (step + 1u == w->steps) {
+            tr->fitness = com_x;
+            tr->vmin = MAX(tr->vmax - tr->vmin, LIT(0.0));
+            tr->vmax = tr->gait_turns * LIT(0.5) / MAX((R)(step + 1u) * w->dt, w->dt);
+            last = 1;
+        }
+        if (ended || last) {
+            const uint32_t fin = orc_f2h(FN(tof)(tr->fitness));
+            uint32_t d = tr->contact_hi;
+            if (step < rung1) { d = (d & 0xffff0000u) | fin; }
+            if (step < rung2) { d = (d & 0x0000ffffu) | (fin << 16u); }
+            tr->contact_hi = d;
+            d = tr->lift_hi;
+            if (step < rung3) { d = (d & 0xffff0000u) | fin; }
+            if (step < rung4) { d = (d & 0x0000ffffu) | (fin << 16u); }
+            tr->lift_hi = d;
+            const uint32_t code = (fell ? 16u : 0u) | (failed ? 32u : 0u) | (tr->screened > LIT(0.0) ? 3u : 0u) |
+                                  (tr->rung_bits & 0x3fc0u) | ((flags & 1u) << 14u);
+            evo_result res;
+            FN(pack_result)(tr, &res);
+            res.previous_center_y = orc_as_float(tr->rung_speed);
+            res.vertical_extremum = orc_as_float(tr->rung_early);
+            res.vertical_trend = orc_as_float(tr->rung_late);
+            res.gait_turns = orc_as_float(tr->ground_hi);
+            res.ground_hi = orc_as_float(code | ((step + 1u < 65535u ? step + 1u : 65535u) << 16u));
+            if (w->record) {
+                tr->kept = res;
+                tr->scoring = 0;
+                tr->limp = 1;
+            } else {
+                *out = res;
+                return 1;
+            }
+        }
+    }
+    if (w->record) {
+        FN(record_frame)(w, b, frames, w->settle + step + 1u);
+        if (step + 1u >= w->steps) {
+            *out = tr->kept;
+            return 1;
+        }
+    }
+    tr->step = step + 1u;
+    return 0;
+}
+
+/* The world of a run from its launch parameters. */
+static void FN(make_world)(FN(World)* w, const evo_params* p, const evo_run* run) {
+    memset(w, 0, sizeof(*w));
+    w->flags = run->flags;
+    w->record = run->record;
+    w->steps = p->steps;
+    w->screen_step = p->screen_step;
+    w->stride = p->stride;
+    w->rate_u = run->rate;
+    w->substeps_u = run->substeps;
+    w->rate = (R)(float)run->rate;
+    w->dt = LIT(1.0) / w->rate;
+    w->inv_h = w->rate * (R)(float)run->substeps;
+    w->h = LIT(1.0) / w->inv_h;
+    w->gravity = p->gravity;
+    w->friction = p->friction;
+    w->terrain = p->terrain;
+    w->muscle_recovery = p->muscle_recovery;
+    w->slope = p->slope;
+    w->wind = p->wind;
+    w->mud = p->mud;
+    w->gaps = p->gaps;
+    w->hurdles = p->hurdles;
+    w->quake = p->quake;
+    w->screen_bar = p->screen_bar;
+    w->screen_bar_young = p->screen_bar_young;
+    w->screen_bar_reshaped = p->screen_bar_reshaped;
+    w->water = p->water;
+    w->patches = p->patches;
+    w->air_sub = p->air_sub;
+    w->inv_muscle_energy = p->inv_muscle_energy;
+    w->brambles = p->brambles;
+    w->rec_h = LIT(0.5) * w->muscle_recovery * w->h;
+    w->damp_share = MIN(w->h * LIT(10.0), LIT(1.0));
+    w->r1 = p->r1;
+    w->r2 = p->r2;
+    w->settle = (200u * run->rate + 119u) / 120u;
+    w->sample = run->rate / 30u > 1u ? run->rate / 30u : 1u;
+    w->head_sample = (uint32_t)((float)run->rate / 60.0f + 0.5f);
+    if (w->head_sample == 0u) { w->head_sample = 1u; }
+}
+
+/* Runs creature cidx to the end of its trial. Returns the steps run, or -1
+ * for a record the simulator cannot run. */
+static long FN(run_creature)(const FN(World)* w, uint32_t cidx, const uint32_t* records, const float* muscles,
+                             const uint32_t* heads, evo_result* out, SIM_FRAME_T* frames) {
+    FN(Body) b;
+    FN(Trial) tr;
+    if (FN(begin_creature)(w, &b, &tr, cidx, records, muscles, heads, frames) != 0) { return -1; }
+    long steps = 0;
+    for (;;) {
+        steps++;
+        if (FN(run_step)(w, &b, &tr, muscles, out, frames)) { break; }
+    }
+    return steps;
+}

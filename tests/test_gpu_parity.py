@@ -1,0 +1,47 @@
+"""GPU against the FP32 oracle, bit for bit, through the C ABI.
+
+The oracle restates shaders/creature.cu with the same arithmetic
+specification as the HIP kernel, so every word of every result must match.
+"""
+import numpy as np
+import pytest
+
+import evolution_simulator_gpu_amd as E
+import oracle
+from tests.helpers import assert_bit_exact, config0_batch, flat_world
+
+pytestmark = pytest.mark.gpu
+
+
+def _both(batch, w, threads=8):
+    p, run = E.run_for(batch, w)
+    want, _, _ = oracle.run_f32(batch, p, run, threads=threads)
+    got = E.simulate(batch, w)
+    return got, want
+
+
+def test_config0_flat_world_is_bit_exact(gpu):
+    batch = config0_batch()
+    got, want = _both(batch, flat_world())
+    assert_bit_exact(got, want, "config 0, flat world, 60 Hz x 8")
+
+
+def test_sixteen_substeps_and_fine_rate(gpu):
+    batch = config0_batch(512)
+    for rate, sub in [(60, 16), (120, 8), (120, 16)]:
+        got, want = _both(batch, flat_world(duration=2.0, substeps=sub, rate=rate))
+        assert_bit_exact(got, want, f"{rate} Hz x {sub}")
+
+
+def test_mid_size_class(gpu):
+    spec = E.synth_spec(1, 2048)
+    batch = E.synth(spec)
+    got, want = _both(batch, flat_world(duration=3.0))
+    assert_bit_exact(got, want, "3-12 nodes, up to 24 constraints")
+
+
+def test_large_bodies_use_looped_class(gpu):
+    spec = E.synth_spec(2, 512)
+    batch = E.synth(spec)
+    got, want = _both(batch, flat_world(duration=2.0))
+    assert_bit_exact(got, want, "3-32 nodes, up to 64 constraints")
