@@ -179,3 +179,65 @@ def simulate(batch: Batch, w: N.WorldConfig, record: bool = False, device: str =
     if record:
         return res, db.frames.cpu().numpy()
     return res
+
+
+def _f32_to_f16(v: float) -> int:
+    return int(np.array([v], dtype=np.float32).astype(np.float16).view(np.uint16)[0])
+
+
+def from_bodies(bodies: list[dict], ids: list[int] | None = None, flags: list[int] | None = None) -> Batch:
+    """Packs bodies given as records into a batch, in the given order.
+
+    Each body is a dict with
+      nodes:   [(mass, radius, friction, x, y, foot), ...]   node 0 is the head
+      bones:   [(pivot, length, parent or None, lo, hi), ...] bone j ends at node j + 1
+      muscles: [dict(a0, a1, b0, b1, anchor_a, anchor_b, hill, cap, inv_capacity, tendon_k, slack,
+                     amplitude, inv_period, phase, duty, inv_duty, inv_complement, stiffness, reset,
+                     sensor=None), ...]
+    and the heads follow src/kernel.rs:554-565 (total and inverse mass summed
+    in node order, the first muscle's period as fp16, the quake hash of `ids`).
+    """
+    lib = N.lib()
+    recs, muss, heads = [], [], []
+    rw = mf = 0
+    for c, body in enumerate(bodies):
+        nodes, bones, muscles = body["nodes"], body["bones"], body.get("muscles", [])
+        assert len(bones) == len(nodes) - 1
+        rec = np.zeros(len(nodes) * N.NODE_WORDS + len(bones) * N.BONE_WORDS, np.uint32)
+        f = rec.view(np.float32)
+        total = np.float32(0.0)
+        for i, (m, r, fr, x, y, foot) in enumerate(nodes):
+            f[i * 6:i * 6 + 5] = [m, r, fr, x, y]
+            rec[i * 6 + 5] = 1 if foot else 0
+            total = np.float32(total + np.float32(m))
+        at = len(nodes) * N.NODE_WORDS
+        for j, (pivot, length, parent, lo, hi) in enumerate(bones):
+            rec[at + j * 6] = pivot
+            f[at + j * 6 + 1] = length
+            rec[at + j * 6 + 2] = N.NONE if parent is None else parent
+            f[at + j * 6 + 3] = lo
+            f[at + j * 6 + 4] = hi
+        mus = np.zeros(max(len(muscles), 0) * N.MUSCLE_WORDS, np.float32)
+        mu = mus.view(np.uint32)
+        for k, m in enumerate(muscles):
+            o = k * N.MUSCLE_WORDS
+            mu[o] = m["a0"] | (m["a1"] << 8) | (m["b0"] << 16) | (m["b1"] << 24)
+            mus[o + 1:o + 8] = [m["anchor_a"], m["anchor_b"], m["hill"], m["cap"], m["inv_capacity"],
+                                m.get("tendon_k", 0.0), m["slack"]]
+            mus[o + 8:o + 16] = [m["amplitude"], m["inv_period"], m["phase"], m["duty"], m["inv_duty"],
+                                 m["inv_complement"], m["stiffness"], m.get("reset", 0.0)]
+            mu[o + 16] = N.NONE if m.get("sensor") is None else m["sensor"]
+        period = (1.0 / muscles[0]["inv_period"]) if muscles else 0.0
+        cid = ids[c] if ids is not None else c + 1
+        fl = flags[c] if flags is not None else 0
+        h = np.array([len(nodes), len(muscles), lib.evo_quake_hash(cid), rw, mf,
+                      np.float32(total).view(np.uint32), np.float32(np.float32(1.0) / total).view(np.uint32),
+                      (_f32_to_f16(period) & 0xFFFF) | (fl << 16)], dtype=np.uint32)
+        recs.append(rec)
+        muss.append(mus)
+        heads.append(h)
+        rw += rec.size
+        mf += mus.size
+    muscles_all = np.concatenate(muss) if mf else np.zeros(4, np.float32)
+    return Batch(np.concatenate(recs), muscles_all.astype(np.float32), np.stack(heads),
+                 np.arange(len(bodies), dtype=np.uint32))

@@ -12,3 +12,9 @@ void oracle_cos_0_pi_n(const float* x, float* out, long n) { for (long i = 0; i 
 void oracle_atan2_n(const float* y, const float* x, float* out, long n) { for (long i = 0; i < n; i++) out[i] = orc_atan2(y[i], x[i]); }
 void oracle_f2h_n(const float* v, uint32_t* out, long n) { for (long i = 0; i < n; i++) out[i] = orc_f2h(v[i]); }
 void oracle_h2f_n(const uint32_t* h, float* out, long n) { for (long i = 0; i < n; i++) out[i] = orc_h2f(h[i]); }
+
+/* The ground and the rhythm of the FP32 and FP64 builds, for their tests. */
+void f32_ground_export(uint32_t flags, const void* params, float x, float amp, float qphase, float* h, float* s);
+void f64_ground_export(uint32_t flags, const void* params, double x, double amp, double qphase, double* h, double* s);
+float f32_rhythm_export(float t, float inv_period, float phase, float offset, float duty, float inv_duty,
+                        float inv_complement);

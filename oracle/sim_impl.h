@@ -872,3 +872,14 @@ static long FN(run_creature)(const FN(World)* w, uint32_t cidx, const uint32_t* 
     }
     return steps;
 }
+
+/* Exported for the oracle's own tests. */
+void FN(ground_export)(uint32_t flags, const evo_params* p, R x, R amp, R qphase, R* h, R* s) {
+    evo_run run = {flags, 60u, 16u, 0u, 0u, 0u};
+    FN(World) w;
+    FN(make_world)(&w, p, &run);
+    FN(ground)(&w, x, amp, qphase, h, s);
+}
+R FN(rhythm_export)(R t, R inv_period, R phase, R offset, R duty, R inv_duty, R inv_complement) {
+    return FN(rhythm)(t, inv_period, phase, offset, duty, inv_duty, inv_complement);
+}
