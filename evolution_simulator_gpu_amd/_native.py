@@ -64,7 +64,7 @@ class Result(C.Structure):
 
 class Run(C.Structure):
     _fields_ = [("flags", C.c_uint32), ("rate", C.c_uint32), ("substeps", C.c_uint32), ("record", C.c_uint32),
-                ("max_nodes", C.c_uint32), ("max_muscles", C.c_uint32)]
+                ("max_nodes", C.c_uint32), ("max_muscles", C.c_uint32), ("no_tendons", C.c_uint32)]
 
 
 class WorldConfig(C.Structure):
@@ -123,7 +123,7 @@ def check_layouts() -> None:
     assert C.sizeof(Rung) == 32
     assert C.sizeof(Params) == 164
     assert C.sizeof(Result) == 76
-    assert C.sizeof(Run) == 24
+    assert C.sizeof(Run) == 28
 
 
 def lib() -> C.CDLL:

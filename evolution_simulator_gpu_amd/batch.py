@@ -33,6 +33,12 @@ class Batch:
     def muscle_counts(self) -> np.ndarray:
         return self.heads[:, 1]
 
+    def tendon_free(self) -> bool:
+        """Whether every muscle's tendon stiffness is 0."""
+        m = self.muscles
+        n = (m.size // N.MUSCLE_WORDS) * N.MUSCLE_WORDS
+        return bool(np.all(m[:n].reshape(-1, N.MUSCLE_WORDS)[:, 6] == 0.0))
+
     def validate(self) -> None:
         bad = C.c_uint32(0)
         rc = N.lib().evo_validate_batch(N.ptr(self.records), self.records.size, N.ptr(self.muscles),
@@ -123,7 +129,26 @@ def run_for(batch: Batch, w: N.WorldConfig, record: bool = False) -> tuple[N.Par
     run.record = 1 if record else 0
     run.max_nodes = int(batch.nodes().max(initial=0))
     run.max_muscles = int(batch.muscle_counts().max(initial=0))
+    run.no_tendons = 1 if batch.tendon_free() else 0
     return p, run
+
+
+# Muscle counts at which the kernel's body-size classes change (csrc/launch.hpp).
+CLASS_MUSCLES = (8, 24)
+
+
+def launch_ranges(batch: Batch) -> list[tuple[int, int, int, int]]:
+    """Contiguous runs of the batch for one launch each: (base, count,
+    max_nodes, max_muscles). A batch packed in (muscles, nodes) order splits
+    at the muscle counts where the body-size classes change, so each run gets
+    the smallest class its bodies fit."""
+    m = batch.muscle_counts().astype(np.int64)
+    n = batch.nodes().astype(np.int64)
+    cls = np.searchsorted(np.array(CLASS_MUSCLES), m, side="left")
+    cuts = np.nonzero(np.diff(cls))[0] + 1
+    starts = np.concatenate([[0], cuts])
+    ends = np.concatenate([cuts, [batch.count]])
+    return [(int(a), int(b - a), int(n[a:b].max()), int(m[a:b].max())) for a, b in zip(starts, ends) if b > a]
 
 
 class DeviceBatch:
@@ -147,17 +172,28 @@ class DeviceBatch:
         fpc = N.lib().evo_settle_frames(rate) + p.steps + 1
         self.frames = torch.zeros(self.batch.count * fpc * p.stride * 2, dtype=torch.float32, device=self.device)
 
-    def launch(self, p: N.Params, run: N.Run, stream=None) -> None:
+    def launch(self, p: N.Params, run: N.Run, stream=None, split: bool = True) -> None:
+        """Simulates creatures [p.base, p.base + p.count) on `stream`, one
+        launch per body-size class when `split` and the range is the whole
+        batch."""
         import torch
 
         if stream is None:
             stream = torch.cuda.current_stream(self.records.device)
-        rc = N.lib().evo_advance_device(
-            self.records.data_ptr(), self.muscles.data_ptr(), self.heads.data_ptr(), self.results.data_ptr(),
-            self.counter.data_ptr(), C.byref(p), C.byref(run),
-            self.frames.data_ptr() if self.frames is not None else None, C.c_void_p(stream.cuda_stream))
-        if rc != N.EVO_OK:
-            raise RuntimeError(f"evo_advance_device failed: {rc}")
+        ranges = [(p.base, p.count, run.max_nodes, run.max_muscles)]
+        if split and not run.record and p.base == 0 and p.count == self.batch.count:
+            ranges = launch_ranges(self.batch)
+        for k, (base, count, maxn, maxm) in enumerate(ranges):
+            q = N.Params.from_buffer_copy(p)
+            q.base, q.count = base, count
+            r = N.Run.from_buffer_copy(run)
+            r.max_nodes, r.max_muscles = maxn, maxm
+            rc = N.lib().evo_advance_device(
+                self.records.data_ptr(), self.muscles.data_ptr(), self.heads.data_ptr(), self.results.data_ptr(),
+                self.counter.data_ptr() + 4 * (k % 4), C.byref(q), C.byref(r),
+                self.frames.data_ptr() if self.frames is not None else None, C.c_void_p(stream.cuda_stream))
+            if rc != N.EVO_OK:
+                raise RuntimeError(f"evo_advance_device failed: {rc}")
 
     def results_numpy(self) -> np.ndarray:
         return self.results.cpu().numpy().view(N.RESULT_DTYPE).reshape(-1)

@@ -57,24 +57,30 @@ hipError_t launch(const uint32_t* records, const float* muscles, const uint4* he
     return hipGetLastError();
 }
 
+template <uint32_t WORLD, bool TENDON>
+Launcher pick_unrolled(const evo_run& run) {
+    constexpr uint32_t R = EVO_RATE, S = EVO_SUB;
+    if (run.max_nodes <= static_cast<uint32_t>(kSmallN) && run.max_muscles <= static_cast<uint32_t>(kSmallM)) {
+        return &launch<Cfg<R, S, WORLD, kSmallN, kSmallM, false, 128, true, TENDON>>;
+    }
+    return &launch<Cfg<R, S, WORLD, kMidN, kMidM, false, 128, true, TENDON>>;
+}
+
 template <uint32_t WORLD>
 Launcher pick_size(const evo_run& run) {
     constexpr uint32_t R = EVO_RATE, S = EVO_SUB;
     const bool known = run.max_nodes != 0u;
-    if (known && run.max_nodes <= static_cast<uint32_t>(kSmallN) && run.max_muscles <= static_cast<uint32_t>(kSmallM)) {
-        return &launch<Cfg<R, S, WORLD, kSmallN, kSmallM, false, 128, true>>;
-    }
     if (known && run.max_nodes <= static_cast<uint32_t>(kMidN) && run.max_muscles <= static_cast<uint32_t>(kMidM)) {
-        return &launch<Cfg<R, S, WORLD, kMidN, kMidM, false, 128, true>>;
+        return run.no_tendons ? pick_unrolled<WORLD, false>(run) : pick_unrolled<WORLD, true>(run);
     }
-    return &launch<Cfg<R, S, WORLD, EVO_MAX_NODES, EVO_MAX_MUSCLES, false, 64, false>>;
+    return &launch<Cfg<R, S, WORLD, EVO_MAX_NODES, EVO_MAX_MUSCLES, false, 64, false, true>>;
 }
 
 }  // namespace
 
 Launcher EVO_PICK(const evo_run& run) {
     if (run.record) {
-        return &launch<Cfg<EVO_RATE, EVO_SUB, kAnyWorld, EVO_MAX_NODES, EVO_MAX_MUSCLES, true, 64, false>>;
+        return &launch<Cfg<EVO_RATE, EVO_SUB, kAnyWorld, EVO_MAX_NODES, EVO_MAX_MUSCLES, true, 64, false, true>>;
     }
     return run.flags == kFlat ? pick_size<kFlat>(run) : pick_size<kAnyWorld>(run);
 }

@@ -6,13 +6,22 @@
 //     lengths, inverse masses and drag; joint topology and ranges; muscle
 //     anchors, Hill factor, caps, tendon; muscle energy and demand) live in
 //     VGPRs. Every loop over nodes, bones, joints and muscles is unrolled to
-//     the body-size class's compile-time maximum and exits per lane at the
-//     creature's own count, so each array element has a fixed register.
+//     the body-size class's compile-time maximum, so each array element has a
+//     fixed register. (The largest class runs real loops and keeps the arrays
+//     in per-lane scratch memory instead.)
 //   - The moving state (position and velocity of each node as one float4,
 //     the muscle force / previous position as one float2, and each muscle's
 //     rhythm offset) lives in LDS, one column per lane. A lane reads the
 //     nodes that its bones, joints and muscles name; with the column layout
 //     the lanes of a wave never meet in a bank whatever nodes they ask for.
+//   - Loops run to the wave's largest count (a scalar bound), and the items a
+//     smaller body lacks are aimed at a spare LDS row, so an item's body has
+//     no per-lane branch and the scheduler can overlap neighbouring items.
+//   - Muscle pulls go into the force table share by share, each a rounded
+//     product added to the node's pair in node order, as the oracle adds them;
+//     the four pairs of a muscle are read together and repeated nodes are
+//     resolved in registers. (LDS float atomic adds give the same sums but
+//     ran at half the speed.)
 //   - The rhythm constants of the muscles stay in the packed records in HBM
 //     and are read once a step.
 //   - The waves of a launch take creatures from a shared counter in chunks
@@ -36,48 +45,17 @@ namespace dev {
 
 constexpr uint32_t kAnyWorld = 0xffffffffu;
 
-// Calls f(integral_constant<int, I>) for I = 0 .. N-1, each call a separate
-// copy of the body, so array indices inside it are compile-time constants.
-template <int I, int N, class F>
-__device__ __forceinline__ void unroll_from(F& f) {
-    if constexpr (I < N) {
-        f(std::integral_constant<int, I>());
-        unroll_from<I + 1, N>(f);
-    }
-}
-template <int N, class F>
-__device__ __forceinline__ void unroll(F&& f) {
-    unroll_from<0, N>(f);
-}
-template <int I, int N, class F>
-__device__ __forceinline__ void unroll_guarded(uint32_t count, F& f) {
-    if constexpr (I < N) {
-        if (static_cast<uint32_t>(I) < count) { f(std::integral_constant<int, I>()); }
-        unroll_guarded<I + 1, N>(count, f);
-    }
-}
-// The loops over a body's nodes, bones and muscles: items START .. count-1
-// of at most N. A small body-size class unrolls them to N guarded copies
-// (constants in registers); a large class runs them as loops (constants in
-// per-lane scratch memory, which a wave reads with one coalesced access per
-// item).
-template <class C, int N, int START, class F>
-__device__ __forceinline__ void loop(uint32_t count, F&& f) {
-    if constexpr (C::unrolled) {
-        unroll_guarded<START, N>(count, f);
-    } else {
-        const uint32_t end = count < static_cast<uint32_t>(N) ? count : static_cast<uint32_t>(N);
-        for (uint32_t i = START; i < end; i++) { f(static_cast<int>(i)); }
-    }
-}
-
-template <uint32_t RATE, uint32_t SUB, uint32_t WORLD, int MAXN_, int MAXM_, bool RECORD, int BLOCK_, bool UNROLLED>
+template <uint32_t RATE, uint32_t SUB, uint32_t WORLD, int MAXN_, int MAXM_, bool RECORD, int BLOCK_, bool UNROLLED,
+          bool TENDON>
 struct Cfg {
     static constexpr bool unrolled = UNROLLED;
+    static constexpr bool tendon = TENDON;
     static constexpr int MAXN = MAXN_;
     static constexpr int MAXB = MAXN_ - 1;
     static constexpr int MAXM = MAXM_;
     static constexpr int BLOCK = BLOCK_;
+    // The spare LDS row that the missing items of a smaller body use.
+    static constexpr uint32_t trash = static_cast<uint32_t>(MAXN_);
     static constexpr bool record = RECORD;
     static constexpr uint32_t world = WORLD;
     static constexpr uint32_t rate_u = RATE;
@@ -109,6 +87,59 @@ __device__ __forceinline__ bool has(uint32_t flags) {
         return (flags & F) != 0u;
     }
 }
+
+// ---------------------------------------------------------------- loops
+
+template <int I, int N, class F>
+__device__ __forceinline__ void unroll_from(F& f) {
+    if constexpr (I < N) {
+        f(std::integral_constant<int, I>());
+        unroll_from<I + 1, N>(f);
+    }
+}
+template <int N, class F>
+__device__ __forceinline__ void unroll(F&& f) {
+    unroll_from<0, N>(f);
+}
+template <int I, int N, int G, class F>
+__device__ __forceinline__ void groups_from(uint32_t bound, F& f) {
+    if constexpr (I < N) {
+        if (static_cast<uint32_t>(I) < bound) {
+            unroll_from<I, (I + G < N ? I + G : N)>(f);
+            groups_from<I + G, N, G>(bound, f);
+        }
+    }
+}
+// Items START .. bound-1 of at most N, where `bound` is the same for the whole
+// wave. An unrolled class runs them as straight-line copies in groups of G
+// behind one scalar branch per group; a looped class runs a scalar loop.
+template <class C, int N, int START, int G, class F>
+__device__ __forceinline__ void items(uint32_t bound, F&& f) {
+    if constexpr (C::unrolled) {
+        groups_from<START, N, G>(bound, f);
+    } else {
+        const uint32_t end = bound < static_cast<uint32_t>(N) ? bound : static_cast<uint32_t>(N);
+        for (uint32_t i = START; i < end; i++) { f(static_cast<int>(i)); }
+    }
+}
+// Items 0 .. N-1 regardless of counts (set-up of a new creature).
+template <class C, int N, class F>
+__device__ __forceinline__ void all_items(F&& f) {
+    if constexpr (C::unrolled) {
+        unroll<N>(f);
+    } else {
+        for (int i = 0; i < N; i++) { f(i); }
+    }
+}
+
+// Marks a register as changed, so the compiler recomputes what depends on it
+// instead of keeping a copy live across the substep loop.
+template <class C>
+__device__ __forceinline__ void opaque(uint32_t& x) {
+    if constexpr (C::unrolled) { asm volatile("" : "+v"(x)); }
+}
+
+// ---------------------------------------------------------------- math
 
 __device__ __forceinline__ float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
 __device__ __forceinline__ float fracf(float x) { return x - floorf(x); }
@@ -180,19 +211,31 @@ __device__ __forceinline__ float ice_at(float x) {
     return s * s * __builtin_fmaf(-2.0f, s, 3.0f);
 }
 
+// ---------------------------------------------------------------- state
+
+// Bone word: bits 0-5 the pivot node, 6-11 the parent bone's tip, 12-17 its
+// pivot, 18-23 the joint's far node, bit 24 set when the hub is also the
+// parent's pivot (a bone at the head turns against the neck), bit 25 set
+// when the bone has a joint (shaders/creature.cu:909-918). Six bits hold every
+// node number and the spare row (at most 32).
+constexpr uint32_t kHasJoint = 1u << 25;
+constexpr uint32_t kAtHead = 1u << 24;
+__device__ __forceinline__ uint32_t topo_word(uint32_t pivot, uint32_t ptip, uint32_t ppiv, uint32_t far_) {
+    return pivot | (ptip << 6u) | (ppiv << 12u) | (far_ << 18u);
+}
+
 // One creature's constants and per-substep muscle state, in registers.
 template <class C>
 struct Body {
     uint32_t nodes, bones, muscles, feet;
     float inv_mass, amp, qphase;
     float nm[C::MAXN], nw[C::MAXN], nr[C::MAXN], nf[C::MAXN];
-    uint32_t bi0[C::MAXB];
+    uint32_t btopo[C::MAXB];
     float blen[C::MAXB], bw0[C::MAXB], bk[C::MAXB], blim[C::MAXB], br0[C::MAXB];
-    uint32_t jtopo[C::MAXB];
     float jlo[C::MAXB], jhi[C::MAXB], jwf[C::MAXB];
     uint32_t mnodes[C::MAXM];
-    float maa[C::MAXM], mab[C::MAXM], mhill[C::MAXM], mcap[C::MAXM], minvcap[C::MAXM], mtk[C::MAXM],
-        mslack[C::MAXM];
+    float maa[C::MAXM], mab[C::MAXM], mhill[C::MAXM], mcap[C::MAXM], minvcap[C::MAXM];
+    float mtk[C::tendon ? C::MAXM : 1], mslack[C::tendon ? C::MAXM : 1];
     float mstore[C::MAXM], mdemand[C::MAXM];
     float mpull[C::record ? C::MAXM : 1];
     float normal_force[C::record ? C::MAXN : 1], friction_force[C::record ? C::MAXN : 1];
@@ -210,6 +253,11 @@ struct Cols {
     __device__ __forceinline__ float& o(uint32_t k) const { return off[k * C::BLOCK]; }
 };
 
+// The wave's largest node, bone and muscle counts: the scalar loop bounds.
+struct Bounds {
+    uint32_t n, b, m;
+};
+
 // The trial so far (shaders/creature.cu:783-826).
 struct Trial {
     float fitness, ground_contact, vmin, vmax, previous_center_y, vertical_extremum, vertical_trend, gait_turns;
@@ -224,6 +272,8 @@ struct Trial {
     evo_result kept;
 };
 
+// ---------------------------------------------------------------- joints
+
 // shaders/creature.cu:320-356
 struct Joint {
     uint32_t tip, far_, hub;
@@ -231,19 +281,19 @@ struct Joint {
 };
 
 template <class C>
-__device__ __forceinline__ Joint joint(const Body<C>& b, const Cols<C>& s, int J, bool with_angle) {
+__device__ __forceinline__ Joint joint(const Body<C>& b, const Cols<C>& s, int j, uint32_t topo, bool with_angle) {
     Joint k;
-    const uint32_t topo = b.jtopo[J];
-    k.tip = static_cast<uint32_t>(J) + 1u;
-    k.hub = topo & 31u;
-    const uint32_t ptip = (topo >> 5u) & 31u, ppiv = (topo >> 10u) & 31u;
-    k.far_ = (topo >> 15u) & 31u;
-    const bool at_head = ((topo >> 20u) & 1u) != 0u;
-    k.lo = b.jlo[J];
-    k.hi = b.jhi[J];
-    k.wt = b.nw[J + 1];
-    k.wf = b.jwf[J];
-    k.wh = b.bw0[J];
+    const bool real = (topo & kHasJoint) != 0u;
+    k.tip = real ? static_cast<uint32_t>(j) + 1u : C::trash;
+    k.hub = topo & 63u;
+    const uint32_t ptip = (topo >> 6u) & 63u, ppiv = (topo >> 12u) & 63u;
+    k.far_ = (topo >> 18u) & 63u;
+    const bool at_head = (topo & kAtHead) != 0u;
+    k.lo = b.jlo[j];
+    k.hi = b.jhi[j];
+    k.wt = b.nw[j + 1];
+    k.wf = b.jwf[j];
+    k.wh = b.bw0[j];
     const float4 pt = s.p(ptip), pp = s.p(ppiv), tt = s.p(k.tip), hh = s.p(k.hub);
     const float ux = pt.x - pp.x, uy = pt.y - pp.y;
     const float vx = tt.x - hh.x, vy = tt.y - hh.y;
@@ -255,7 +305,8 @@ __device__ __forceinline__ Joint joint(const Body<C>& b, const Cols<C>& s, int J
         const float mid = 0.5f * (k.lo + k.hi);
         k.angle = mid + wrapf(turn - mid);
     }
-    const float ivv = 1.0f / vv, iuu = 1.0f / uu;
+    const float inv = 1.0f / (uu * vv);
+    const float ivv = uu * inv, iuu = vv * inv;
     k.gtx = -vy * ivv;
     k.gty = vx * ivv;
     const float gux = -uy * iuu, guy = ux * iuu;
@@ -274,70 +325,96 @@ __device__ __forceinline__ float joint_weight(const Joint& k) {
 }
 
 // Pushes the positions (VEL false) or velocities (VEL true) of the joint's
-// three nodes, tip first, then far, then hub, each a read-modify-write so that
-// a node named twice takes both pushes in order.
+// three nodes. Tip, far and hub of a joint are always three different nodes
+// (the tip is node j + 1, the hub and the far node are nodes of earlier
+// bones, and the far node is not the hub), so the three reads go out
+// together and the three writes follow. Nothing changes where `on` is false
+// (the missing joints of a smaller body name the spare row).
 template <class C, bool VEL>
-__device__ __forceinline__ void joint_push(const Cols<C>& s, const Joint& k, float lambda) {
+__device__ __forceinline__ void joint_push(const Cols<C>& s, const Joint& k, float lambda, bool on) {
     const float st = k.wt * lambda, sf = k.wf * lambda, sh = k.wh * lambda;
-    {
-        float4& q = s.p(k.tip);
-        float4 v = q;
-        if (VEL) { v.z = __builtin_fmaf(st, k.gtx, v.z); v.w = __builtin_fmaf(st, k.gty, v.w); }
-        else { v.x = __builtin_fmaf(st, k.gtx, v.x); v.y = __builtin_fmaf(st, k.gty, v.y); }
-        q = v;
+    float4& rt = s.p(k.tip);
+    float4& rf = s.p(k.far_);
+    float4& rh = s.p(k.hub);
+    float4 vt = rt, vf = rf, vh = rh;
+    if (VEL) {
+        vt.z = on ? __builtin_fmaf(st, k.gtx, vt.z) : vt.z;
+        vt.w = on ? __builtin_fmaf(st, k.gty, vt.w) : vt.w;
+        vf.z = on ? __builtin_fmaf(sf, k.gfx, vf.z) : vf.z;
+        vf.w = on ? __builtin_fmaf(sf, k.gfy, vf.w) : vf.w;
+        vh.z = on ? __builtin_fmaf(sh, k.ghx, vh.z) : vh.z;
+        vh.w = on ? __builtin_fmaf(sh, k.ghy, vh.w) : vh.w;
+    } else {
+        vt.x = on ? __builtin_fmaf(st, k.gtx, vt.x) : vt.x;
+        vt.y = on ? __builtin_fmaf(st, k.gty, vt.y) : vt.y;
+        vf.x = on ? __builtin_fmaf(sf, k.gfx, vf.x) : vf.x;
+        vf.y = on ? __builtin_fmaf(sf, k.gfy, vf.y) : vf.y;
+        vh.x = on ? __builtin_fmaf(sh, k.ghx, vh.x) : vh.x;
+        vh.y = on ? __builtin_fmaf(sh, k.ghy, vh.y) : vh.y;
     }
-    {
-        float4& q = s.p(k.far_);
-        float4 v = q;
-        if (VEL) { v.z = __builtin_fmaf(sf, k.gfx, v.z); v.w = __builtin_fmaf(sf, k.gfy, v.w); }
-        else { v.x = __builtin_fmaf(sf, k.gfx, v.x); v.y = __builtin_fmaf(sf, k.gfy, v.y); }
-        q = v;
-    }
-    {
-        float4& q = s.p(k.hub);
-        float4 v = q;
-        if (VEL) { v.z = __builtin_fmaf(sh, k.ghx, v.z); v.w = __builtin_fmaf(sh, k.ghy, v.w); }
-        else { v.x = __builtin_fmaf(sh, k.ghx, v.x); v.y = __builtin_fmaf(sh, k.ghy, v.y); }
-        q = v;
-    }
+    rt = vt;
+    rf = vf;
+    rh = vh;
 }
+
+// ---------------------------------------------------------------- substep
 
 // shaders/creature.cu:442-451 — the step's demand of every muscle.
 template <class C>
 __device__ __forceinline__ void muscle_demands(Body<C>& b, const Cols<C>& s, const float* __restrict__ mus, float t,
-                                               bool limp) {
-    loop<C, C::MAXM, 0>(b.muscles, [&](auto kc_) {
-        const int k = kc_;
-        const float4* q = reinterpret_cast<const float4*>(mus + k * EVO_MUSCLE_WORDS);
-        const float4 r2 = q[2], r3 = q[3];
-        const float offset = s.o(k);
-        const float before = rhythm(t, r2.y, r2.z, offset, r2.w, r3.x, r3.y);
-        const float after = rhythm(t + C::dt, r2.y, r2.z, offset, r2.w, r3.x, r3.y);
-        const float shortening = r2.x * (after - before) * C::rate;
-        b.mdemand[k] = limp ? 0.0f : fmaxf(shortening * r3.z * 0.25f, 0.0f);
+                                               bool limp, const Bounds& wb) {
+    items<C, C::MAXM, 0, 1>(wb.m, [&](auto kc) {
+        const int k = kc;
+        if (static_cast<uint32_t>(k) < b.muscles) {
+            const float4* q = reinterpret_cast<const float4*>(mus + k * EVO_MUSCLE_WORDS);
+            const float4 r2 = q[2], r3 = q[3];
+            const float offset = s.o(k);
+            const float before = rhythm(t, r2.y, r2.z, offset, r2.w, r3.x, r3.y);
+            const float after = rhythm(t + C::dt, r2.y, r2.z, offset, r2.w, r3.x, r3.y);
+            const float shortening = r2.x * (after - before) * C::rate;
+            b.mdemand[k] = limp ? 0.0f : fmaxf(shortening * r3.z * 0.25f, 0.0f);
+        }
     });
 }
 
-// Adds (gx, gy) to node i's force pair.
+// Adds the four shares of a muscle's pull to the force pairs of its nodes
+// a0, a1, b0, b1, in that order, with the sums of four read-add-write steps:
+// the four pairs are read together, a node named by an earlier slot starts
+// from that slot's new sum, and the four writes follow in slot order, so the
+// last write to a node holds all its shares.
 template <class C>
-__device__ __forceinline__ void add_force(const Cols<C>& s, uint32_t i, float gx, float gy, float share) {
-    float2& q = s.f(i);
-    float2 v = q;
-    v.x = __builtin_fmaf(gx, share, v.x);
-    v.y = __builtin_fmaf(gy, share, v.y);
-    q = v;
+__device__ __forceinline__ void add_shares(const Cols<C>& s, uint32_t a0, uint32_t a1, uint32_t b0, uint32_t b1,
+                                           float2 c0, float2 c1, float2 c2, float2 c3) {
+    float2& r0 = s.f(a0);
+    float2& r1 = s.f(a1);
+    float2& r2 = s.f(b0);
+    float2& r3 = s.f(b1);
+    const float2 v0 = r0, v1 = r1, v2 = r2, v3 = r3;
+    const bool e10 = a1 == a0, e21 = b0 == a1, e20 = b0 == a0, e32 = b1 == b0, e31 = b1 == a1, e30 = b1 == a0;
+    float2 n0, n1, n2, n3;
+    n0.x = v0.x + c0.x;
+    n0.y = v0.y + c0.y;
+    n1.x = (e10 ? n0.x : v1.x) + c1.x;
+    n1.y = (e10 ? n0.y : v1.y) + c1.y;
+    n2.x = (e21 ? n1.x : e20 ? n0.x : v2.x) + c2.x;
+    n2.y = (e21 ? n1.y : e20 ? n0.y : v2.y) + c2.y;
+    n3.x = (e32 ? n2.x : e31 ? n1.x : e30 ? n0.x : v3.x) + c3.x;
+    n3.y = (e32 ? n2.y : e31 ? n1.y : e30 ? n0.y : v3.y) + c3.y;
+    r0 = n0;
+    r1 = n1;
+    r2 = n2;
+    r3 = n3;
 }
 
 // shaders/creature.cu:455-508 — the muscles' pull and their energy stores.
 template <class C>
-__device__ __forceinline__ void muscle_forces(Body<C>& b, const Cols<C>& s, float rec_h, float inv_me) {
-    loop<C, C::MAXN, 0>(b.nodes, [&](auto ic_) {
-        const int i = ic_;
-        s.f(i) = make_float2(0.0f, 0.0f);
-    });
-    loop<C, C::MAXM, 0>(b.muscles, [&](auto kc_) {
-        const int k = kc_;
-        const uint32_t nodes = b.mnodes[k];
+__device__ __forceinline__ void muscle_forces(Body<C>& b, const Cols<C>& s, float rec_h, float inv_me,
+                                              const Bounds& wb) {
+    items<C, C::MAXN, 0, 4>(wb.n, [&](auto ic) { s.f(static_cast<int>(ic)) = make_float2(0.0f, 0.0f); });
+    items<C, C::MAXM, 0, 2>(wb.m, [&](auto kc) {
+        const int k = kc;
+        uint32_t nodes = b.mnodes[k];
+        opaque<C>(nodes);
         const uint32_t a0 = nodes & 255u, a1 = (nodes >> 8u) & 255u, b0 = (nodes >> 16u) & 255u, b1 = nodes >> 24u;
         const float aa = b.maa[k], ab = b.mab[k];
         const float4 qa0 = s.p(a0), qa1 = s.p(a1), qb0 = s.p(b0), qb1 = s.p(b1);
@@ -356,38 +433,48 @@ __device__ __forceinline__ void muscle_forces(Body<C>& b, const Cols<C>& s, floa
         const float lengthening = __builtin_fmaf(bvx - avx, ex, (bvy - avy) * ey);
         const float store = b.mstore[k];
         float drive = b.mdemand[k] * store;
-        if (b.mhill[k] > 0.0f) { drive *= clampf(__builtin_fmaf(lengthening, b.mhill[k], 1.0f), 0.0f, 1.0f); }
+        const float hill = b.mhill[k];
+        const float scaled = drive * clampf(__builtin_fmaf(lengthening, hill, 1.0f), 0.0f, 1.0f);
+        drive = hill > 0.0f ? scaled : drive;
         const float cap = b.mcap[k];
         const float active = clampf(__builtin_fmaf(lengthening, 0.15f, drive), -cap, cap);
         const float work = fminf(drive, cap) * fmaxf(-lengthening, 0.0f) * C::h;
         b.mstore[k] = clampf(__builtin_fmaf(rec_h, 1.0f - store, store - work * b.minvcap[k] * inv_me), 0.0f, 1.0f);
-        const float pull = __builtin_fmaf(b.mtk[k], fmaxf(len - b.mslack[k], 0.0f), active);
+        float pull;
+        if constexpr (C::tendon) {
+            pull = __builtin_fmaf(b.mtk[k], fmaxf(len - b.mslack[k], 0.0f), active);
+        } else {
+            // The same value as a tendon of stiffness 0: 0 * len + active,
+            // rounded once (len is at least 1e-6, so 0 * len is +0 unless len
+            // is not finite).
+            pull = __builtin_fmaf(0.0f, len, active);
+        }
         if constexpr (C::record) { b.mpull[k] = pull; }
         const float fx = ex * pull, fy = ey * pull;
-        add_force<C>(s, a0, fx, fy, 1.0f - aa);
-        add_force<C>(s, a1, fx, fy, aa);
-        add_force<C>(s, b0, -fx, -fy, 1.0f - ab);
-        add_force<C>(s, b1, -fx, -fy, ab);
+        const float ra = 1.0f - aa, rb = 1.0f - ab;
+        add_shares<C>(s, a0, a1, b0, b1, make_float2(fx * ra, fy * ra), make_float2(fx * aa, fy * aa),
+                      make_float2(-(fx * rb), -(fy * rb)), make_float2(-(fx * ab), -(fy * ab)));
     });
 }
 
 // shaders/creature.cu:511-546
 template <class C>
-__device__ __forceinline__ float2 node_force(const Body<C>& b, int I, const float4& q, const evo_params& p,
+__device__ __forceinline__ float2 node_force(const Body<C>& b, int i, const float4& q, const evo_params& p,
                                              uint32_t flags) {
-    const float m = b.nm[I];
+    const float m = b.nm[i];
     float fx = 0.0f, fy = -p.gravity * m;
     if (has<C, EVO_WIND>(flags)) { fx = __builtin_fmaf(p.wind, m, fx); }
     if (has<C, EVO_MUD>(flags) || has<C, EVO_BRAMBLES>(flags)) {
         const float2 g = ground<C>(q.x, b.amp, b.qphase, p, flags);
-        const float dry = (q.y - g.x) / sqrtf(__builtin_fmaf(g.y, g.y, 1.0f)) - b.nr[I];
+        const float dry = (q.y - g.x) / sqrtf(__builtin_fmaf(g.y, g.y, 1.0f)) - b.nr[i];
         if (has<C, EVO_MUD>(flags)) { fx -= m * 2.0f * (clampf(-dry, 0.0f, p.mud) * (1.0f / 0.1f)) * q.z; }
         if (has<C, EVO_BRAMBLES>(flags)) {
-            if (((b.feet >> I) & 1u) == 0u && dry < 0.01f) { fx -= m * p.brambles * q.z; }
+            const float held = fx - m * p.brambles * q.z;
+            fx = (((b.feet >> i) & 1u) == 0u && dry < 0.01f) ? held : fx;
         }
     }
     if (has<C, EVO_WATER>(flags)) {
-        const float wet = clampf((p.water - (q.y - b.nr[I])) / (2.0f * b.nr[I]), 0.0f, 1.0f);
+        const float wet = clampf((p.water - (q.y - b.nr[i])) / (2.0f * b.nr[i]), 0.0f, 1.0f);
         fy = __builtin_fmaf(0.7f * m * p.gravity, wet, fy);
     }
     return make_float2(fx, fy);
@@ -395,10 +482,13 @@ __device__ __forceinline__ float2 node_force(const Body<C>& b, int I, const floa
 
 // shaders/creature.cu:553-593
 template <class C>
-__device__ __forceinline__ void bone_drag(const Body<C>& b, const Cols<C>& s, const evo_params& p, uint32_t flags) {
-    loop<C, C::MAXB, 0>(b.bones, [&](auto jc_) {
-        const int j = jc_;
-        const uint32_t i0 = b.bi0[j];
+__device__ __forceinline__ void bone_drag(const Body<C>& b, const Cols<C>& s, const evo_params& p, uint32_t flags,
+                                          const Bounds& wb) {
+    items<C, C::MAXB, 0, 2>(wb.b, [&](auto jc) {
+        const int j = jc;
+        uint32_t topo = b.btopo[j];
+        opaque<C>(topo);
+        const uint32_t i0 = topo & 63u;
         float4& q0r = s.p(i0);
         float4& q1r = s.p(j + 1);
         float4 q0 = q0r, q1 = q1r;
@@ -424,19 +514,21 @@ __device__ __forceinline__ void bone_drag(const Body<C>& b, const Cols<C>& s, co
         const float s0 = C::half_h * b.bw0[j], s1 = C::half_h * b.nw[j + 1];
         q0.z = __builtin_fmaf(s0, dfx, q0.z);
         q0.w = __builtin_fmaf(s0, dfy, q0.w);
-        q0r = q0;
         q1.z = __builtin_fmaf(s1, dfx, q1.z);
         q1.w = __builtin_fmaf(s1, dfy, q1.w);
+        q0r = q0;
         q1r = q1;
     });
 }
 
 // shaders/creature.cu:597-613
 template <class C>
-__device__ __forceinline__ void solve_bones(const Body<C>& b, const Cols<C>& s) {
-    loop<C, C::MAXB, 0>(b.bones, [&](auto jc_) {
-        const int j = jc_;
-        const uint32_t i0 = b.bi0[j];
+__device__ __forceinline__ void solve_bones(const Body<C>& b, const Cols<C>& s, const Bounds& wb) {
+    items<C, C::MAXB, 0, 1>(wb.b, [&](auto jc) {
+        const int j = jc;
+        uint32_t topo = b.btopo[j];
+        opaque<C>(topo);
+        const uint32_t i0 = topo & 63u;
         float4& q0r = s.p(i0);
         float4& q1r = s.p(j + 1);
         float4 q0 = q0r, q1 = q1r;
@@ -456,93 +548,94 @@ __device__ __forceinline__ void solve_bones(const Body<C>& b, const Cols<C>& s) 
 
 // shaders/creature.cu:618-634
 template <class C>
-__device__ __forceinline__ void solve_joints(const Body<C>& b, const Cols<C>& s) {
-    loop<C, C::MAXB, 1>(b.bones, [&](auto jc_) {
-        const int j = jc_;
-        {
-            if (b.jtopo[j] == EVO_NONE) { return; }
-            const Joint k = joint<C>(b, s, j, true);
-            float error = 0.0f;
-            if (k.angle < k.lo) { error = k.angle - k.lo; }
-            if (k.angle > k.hi) { error = k.angle - k.hi; }
-            if (error == 0.0f) { return; }
+__device__ __forceinline__ void solve_joints(const Body<C>& b, const Cols<C>& s, const Bounds& wb) {
+    items<C, C::MAXB, 1, 1>(wb.b, [&](auto jc) {
+        const int j = jc;
+        uint32_t topo = b.btopo[j];
+        opaque<C>(topo);
+        const Joint k = joint<C>(b, s, j, topo, true);
+        float error = 0.0f;
+        if (k.angle < k.lo) { error = k.angle - k.lo; }
+        if (k.angle > k.hi) { error = k.angle - k.hi; }
+        if ((topo & kHasJoint) != 0u && error != 0.0f) {
             const float w = joint_weight(k);
-            if (w > 0.0f) { joint_push<C, false>(s, k, -error / w); }
+            if (w > 0.0f) { joint_push<C, false>(s, k, -error / w, true); }
         }
     });
 }
 
 // shaders/creature.cu:683-702
 template <class C>
-__device__ __forceinline__ void damp_joints(const Body<C>& b, const Cols<C>& s) {
-    loop<C, C::MAXB, 1>(b.bones, [&](auto jc_) {
-        const int j = jc_;
-        {
-            if (b.jtopo[j] == EVO_NONE) { return; }
-            const Joint k = joint<C>(b, s, j, false);
-            const float4 vt = s.p(k.tip), vf = s.p(k.far_), vh = s.p(k.hub);
-            float rate = k.gtx * vt.z;
-            rate = __builtin_fmaf(k.gty, vt.w, rate);
-            rate = __builtin_fmaf(k.gfx, vf.z, rate);
-            rate = __builtin_fmaf(k.gfy, vf.w, rate);
-            rate = __builtin_fmaf(k.ghx, vh.z, rate);
-            rate = __builtin_fmaf(k.ghy, vh.w, rate);
-            const float w = joint_weight(k);
-            if (w > 0.0f) { joint_push<C, true>(s, k, (-C::damp_share * rate) / w); }
-        }
+__device__ __forceinline__ void damp_joints(const Body<C>& b, const Cols<C>& s, const Bounds& wb) {
+    items<C, C::MAXB, 1, 1>(wb.b, [&](auto jc) {
+        const int j = jc;
+        uint32_t topo = b.btopo[j];
+        opaque<C>(topo);
+        const Joint k = joint<C>(b, s, j, topo, false);
+        const float4 vt = s.p(k.tip), vf = s.p(k.far_), vh = s.p(k.hub);
+        float rate = k.gtx * vt.z;
+        rate = __builtin_fmaf(k.gty, vt.w, rate);
+        rate = __builtin_fmaf(k.gfx, vf.z, rate);
+        rate = __builtin_fmaf(k.gfy, vf.w, rate);
+        rate = __builtin_fmaf(k.ghx, vh.z, rate);
+        rate = __builtin_fmaf(k.ghy, vh.w, rate);
+        const float w = joint_weight(k);
+        const bool on = (topo & kHasJoint) != 0u && w > 0.0f;
+        joint_push<C, true>(s, k, (-C::damp_share * rate) / w, on);
     });
 }
 
 // shaders/creature.cu:746-759
 template <class C>
-__device__ __forceinline__ uint64_t broken_joints(const Body<C>& b, const Cols<C>& s) {
+__device__ __forceinline__ uint64_t broken_joints(const Body<C>& b, const Cols<C>& s, const Bounds& wb) {
     uint64_t bits = 0u;
-    loop<C, C::MAXB, 1>(b.bones, [&](auto jc_) {
-        const int j = jc_;
-        {
-            if (b.jtopo[j] == EVO_NONE) { return; }
-            const Joint k = joint<C>(b, s, j, true);
-            if (k.angle < k.lo - 0.5f || k.angle > k.hi + 0.5f) { bits |= 1ull << j; }
+    items<C, C::MAXB, 1, 1>(wb.b, [&](auto jc) {
+        const int j = jc;
+        const uint32_t topo = b.btopo[j];
+        const Joint k = joint<C>(b, s, j, topo, true);
+        if ((topo & kHasJoint) != 0u && (k.angle < k.lo - 0.5f || k.angle > k.hi + 0.5f)) {
+            bits |= 1ull << j;
         }
     });
     return bits;
 }
 
-// shaders/creature.cu:640-679 plus the velocity of :733-740, for node I.
+// shaders/creature.cu:640-679 plus the velocity of :733-740, for node i.
 template <class C>
-__device__ __forceinline__ void ground_and_velocity(Body<C>& b, const Cols<C>& s, int I, const evo_params& p,
+__device__ __forceinline__ void ground_and_velocity(Body<C>& b, const Cols<C>& s, int i, const evo_params& p,
                                                     uint32_t flags) {
-    float4& qr = s.p(I);
+    float4& qr = s.p(i);
     float4 q = qr;
-    const float2 prev = s.f(I);
+    const float2 prev = s.f(i);
     if (has<C, EVO_GROUND>(flags)) {
         const float2 g = ground<C>(q.x, b.amp, b.qphase, p, flags);
         const float secant = sqrtf(__builtin_fmaf(g.y, g.y, 1.0f));
         const float inv_sec = 1.0f / secant;
         const float nx = -g.y * inv_sec, ny = inv_sec;
-        const float dry = __builtin_fmaf(q.y - g.x, inv_sec, -b.nr[I]);
+        const float dry = __builtin_fmaf(q.y - g.x, inv_sec, -b.nr[i]);
         float depth, mu;
         if (has<C, EVO_MUD>(flags)) {
             depth = -(dry + p.mud);
             const float sink = clampf(-dry, 0.0f, p.mud) * (1.0f / 0.1f);
-            mu = b.nf[I] * p.friction * __builtin_fmaf(2.0f, sink, 1.0f) * __builtin_fmaf(2.0f, sink, 1.0f);
+            mu = b.nf[i] * p.friction * __builtin_fmaf(2.0f, sink, 1.0f) * __builtin_fmaf(2.0f, sink, 1.0f);
         } else {
             depth = -dry;
-            mu = b.nf[I] * p.friction;
+            mu = b.nf[i] * p.friction;
         }
-        if (depth > 0.0f) {
-            if (has<C, EVO_ICE>(flags)) { mu *= __builtin_fmaf(-p.patches, ice_at(q.x), 1.0f); }
-            q.x = __builtin_fmaf(nx, depth, q.x);
-            q.y = __builtin_fmaf(ny, depth, q.y);
-            const float tx = ny, ty = -nx;
-            const float slide = __builtin_fmaf(q.x - prev.x, tx, (q.y - prev.y) * ty);
-            const float budget = mu * depth;
-            const float back = fabsf(slide) <= budget ? slide : copysignf(budget, slide);
-            q.x = __builtin_fmaf(-tx, back, q.x);
-            q.y = __builtin_fmaf(-ty, back, q.y);
-            if constexpr (C::record) {
-                b.normal_force[I] += b.nm[I] * depth * C::inv_h * C::inv_h;
-                b.friction_force[I] += b.nm[I] * back * C::inv_h * C::inv_h;
+        if (has<C, EVO_ICE>(flags)) { mu *= __builtin_fmaf(-p.patches, ice_at(q.x), 1.0f); }
+        const bool in = depth > 0.0f;
+        const float x1 = __builtin_fmaf(nx, depth, q.x);
+        const float y1 = __builtin_fmaf(ny, depth, q.y);
+        const float tx = ny, ty = -nx;
+        const float slide = __builtin_fmaf(x1 - prev.x, tx, (y1 - prev.y) * ty);
+        const float budget = mu * depth;
+        const float back = fabsf(slide) <= budget ? slide : copysignf(budget, slide);
+        q.x = in ? __builtin_fmaf(-tx, back, x1) : q.x;
+        q.y = in ? __builtin_fmaf(-ty, back, y1) : q.y;
+        if constexpr (C::record) {
+            if (in) {
+                b.normal_force[i] += b.nm[i] * depth * C::inv_h * C::inv_h;
+                b.friction_force[i] += b.nm[i] * back * C::inv_h * C::inv_h;
             }
         }
     }
@@ -552,20 +645,20 @@ __device__ __forceinline__ void ground_and_velocity(Body<C>& b, const Cols<C>& s
 }
 
 template <class C>
-__device__ __forceinline__ void integrate(const Body<C>& b, const Cols<C>& s, int I, const evo_params& p,
+__device__ __forceinline__ void integrate(const Body<C>& b, const Cols<C>& s, int i, const evo_params& p,
                                           uint32_t flags) {
-    float4& qr = s.p(I);
+    float4& qr = s.p(i);
     float4 q = qr;
-    const float2 fm = s.f(I);
-    const float2 f = node_force<C>(b, I, q, p, flags);
-    const float wi = b.nw[I];
+    const float2 fm = s.f(i);
+    const float2 f = node_force<C>(b, i, q, p, flags);
+    const float wi = b.nw[i];
     q.z = __builtin_fmaf(C::h * (f.x + fm.x), wi, q.z);
     q.w = __builtin_fmaf(C::h * (f.y + fm.y), wi, q.w);
     if (has<C, EVO_AIR>(flags)) {
         q.z *= p.air_sub;
         q.w *= p.air_sub;
     }
-    s.f(I) = make_float2(q.x, q.y);
+    s.f(i) = make_float2(q.x, q.y);
     q.x = __builtin_fmaf(C::h, q.z, q.x);
     q.y = __builtin_fmaf(C::h, q.w, q.y);
     qr = q;
@@ -574,38 +667,37 @@ __device__ __forceinline__ void integrate(const Body<C>& b, const Cols<C>& s, in
 // shaders/creature.cu:705-742 — one substep.
 template <class C>
 __device__ __forceinline__ void substep(Body<C>& b, const Cols<C>& s, const evo_params& p, uint32_t flags,
-                                       float rec_h) {
-    muscle_forces<C>(b, s, rec_h, p.inv_muscle_energy);
-    bone_drag<C>(b, s, p, flags);
-    loop<C, C::MAXN, 0>(b.nodes, [&](auto ic_) { integrate<C>(b, s, ic_, p, flags); });
-    solve_bones<C>(b, s);
-    solve_joints<C>(b, s);
-    loop<C, C::MAXN, 0>(b.nodes, [&](auto ic_) { ground_and_velocity<C>(b, s, ic_, p, flags); });
-    damp_joints<C>(b, s);
+                                       float rec_h, const Bounds& wb) {
+    muscle_forces<C>(b, s, rec_h, p.inv_muscle_energy, wb);
+    bone_drag<C>(b, s, p, flags, wb);
+    items<C, C::MAXN, 0, 2>(wb.n, [&](auto ic) { integrate<C>(b, s, ic, p, flags); });
+    solve_bones<C>(b, s, wb);
+    solve_joints<C>(b, s, wb);
+    items<C, C::MAXN, 0, 2>(wb.n, [&](auto ic) { ground_and_velocity<C>(b, s, ic, p, flags); });
+    damp_joints<C>(b, s, wb);
 }
+
+// ---------------------------------------------------------------- per step
 
 // shaders/creature.cu:767-779 — frame t of a recording.
 template <class C>
 __device__ __forceinline__ void record_frame(const Body<C>& b, const Cols<C>& s, float2* __restrict__ frames,
-                                             uint32_t t, uint32_t stride) {
+                                             uint32_t t, uint32_t stride, const Bounds& wb) {
     if constexpr (C::record) {
         float2* f = frames + static_cast<size_t>(t) * stride;
-        loop<C, C::MAXN, 0>(b.nodes, [&](auto ic_) {
-            const int i = ic_;
+        for (uint32_t i = 0; i < b.nodes; i++) {
             const float4 q = s.p(i);
             f[i] = make_float2(q.x, q.y);
             f[EVO_MAX_NODES + b.muscles + i] = make_float2(b.normal_force[i], b.friction_force[i]);
-        });
-        loop<C, C::MAXM, 0>(b.muscles, [&](auto kc_) {
-            const int k = kc_;
-            f[EVO_MAX_NODES + k] = make_float2(b.mstore[k], b.mpull[k]);
-        });
-        const uint64_t broken = broken_joints<C>(b, s);
+        }
+        for (uint32_t k = 0; k < b.muscles; k++) { f[EVO_MAX_NODES + k] = make_float2(b.mstore[k], b.mpull[k]); }
+        const uint64_t broken = broken_joints<C>(b, s, wb);
         f[stride - 1u] = make_float2(bits_f(static_cast<uint32_t>(broken)), bits_f(static_cast<uint32_t>(broken >> 32)));
     }
 }
 
-// shaders/creature.cu:845-950 — the lane's new creature.
+// shaders/creature.cu:845-950 — the lane's new creature. Items beyond the
+// body's counts are aimed at the spare row.
 template <class C>
 __device__ __forceinline__ void begin_creature(Body<C>& b, Trial& tr, const Cols<C>& s, uint32_t cidx,
                                                const uint32_t* __restrict__ records,
@@ -631,70 +723,99 @@ __device__ __forceinline__ void begin_creature(Body<C>& b, Trial& tr, const Cols
         b.amp = p.terrain;
     }
     b.feet = 0u;
-    loop<C, C::MAXN, 0>(b.nodes, [&](auto ic_) {
-        const int i = ic_;
-        const uint32_t* r = rec + i * EVO_NODE_WORDS;
-        b.nm[i] = bits_f(r[0]);
-        b.nw[i] = 1.0f / b.nm[i];
-        b.nr[i] = bits_f(r[1]);
-        b.nf[i] = bits_f(r[2]);
-        s.p(i) = make_float4(bits_f(r[3]), bits_f(r[4]), 0.0f, 0.0f);
-        b.feet |= (r[5] & 1u) << i;
+    all_items<C, C::MAXN>([&](auto ic) {
+        const int i = ic;
+        if (static_cast<uint32_t>(i) < b.nodes) {
+            const uint32_t* r = rec + i * EVO_NODE_WORDS;
+            b.nm[i] = bits_f(r[0]);
+            b.nw[i] = 1.0f / b.nm[i];
+            b.nr[i] = bits_f(r[1]);
+            b.nf[i] = bits_f(r[2]);
+            s.p(i) = make_float4(bits_f(r[3]), bits_f(r[4]), 0.0f, 0.0f);
+            b.feet |= (r[5] & 1u) << i;
+        } else {
+            b.nm[i] = 1.0f;
+            b.nw[i] = 1.0f;
+            b.nr[i] = 1.0f;
+            b.nf[i] = 0.0f;
+        }
         if constexpr (C::record) {
             b.normal_force[i] = 0.0f;
             b.friction_force[i] = 0.0f;
         }
     });
     const uint32_t* bone_rec = rec + b.nodes * EVO_NODE_WORDS;
-    loop<C, C::MAXB, 0>(b.bones, [&](auto jc_) {
-        const int j = jc_;
-        const uint32_t* r = bone_rec + j * EVO_BONE_WORDS;
-        const uint32_t i0 = r[0];
-        // Constants of the pivot node come from its record: the registers
-        // of the node arrays cannot be indexed by a per-lane number.
-        const float m0 = bits_f(rec[i0 * EVO_NODE_WORDS]);
-        const float r0 = bits_f(rec[i0 * EVO_NODE_WORDS + 1]);
-        const float length = bits_f(r[1]);
-        const float width = r0 + b.nr[j + 1];
-        const float bone_mass = m0 + b.nm[j + 1];
-        b.bi0[j] = i0;
-        b.blen[j] = length;
-        b.bw0[j] = 1.0f / m0;
-        b.bk[j] = 0.6f * length * width;
-        b.blim[j] = 0.5f * bone_mass * C::inv_h;
-        b.br0[j] = r0;
-        b.jtopo[j] = EVO_NONE;
-        const uint32_t pj = r[2];
-        if (j > 0 && pj != EVO_NONE) {
-            const uint32_t hub = i0, parent_pivot = bone_rec[pj * EVO_BONE_WORDS];
-            const bool at_head = hub == parent_pivot;
-            const uint32_t far_ = at_head ? pj + 1u : parent_pivot;
-            b.jtopo[j] = hub | ((pj + 1u) << 5u) | (parent_pivot << 10u) | (far_ << 15u) | ((at_head ? 1u : 0u) << 20u);
-            b.jlo[j] = bits_f(r[3]);
-            b.jhi[j] = bits_f(r[4]);
-            b.jwf[j] = 1.0f / bits_f(rec[far_ * EVO_NODE_WORDS]);
-        } else {
-            b.jlo[j] = 0.0f;
-            b.jhi[j] = 0.0f;
-            b.jwf[j] = 0.0f;
+    all_items<C, C::MAXB>([&](auto jc) {
+        const int j = jc;
+        uint32_t topo = topo_word(C::trash, C::trash, C::trash, C::trash);
+        float length = 1.0f, w0 = 1.0f, k_air = 0.0f, limit = 0.0f, r0 = 1.0f;
+        float lo = 0.0f, hi = 0.0f, wf = 0.0f;
+        if (static_cast<uint32_t>(j) < b.bones) {
+            const uint32_t* r = bone_rec + j * EVO_BONE_WORDS;
+            const uint32_t i0 = r[0];
+            // Constants of the pivot node come from its record: the node
+            // arrays live in registers that a per-lane number cannot index.
+            const float m0 = bits_f(rec[i0 * EVO_NODE_WORDS]);
+            r0 = bits_f(rec[i0 * EVO_NODE_WORDS + 1]);
+            length = bits_f(r[1]);
+            const float width = r0 + b.nr[j + 1];
+            const float bone_mass = m0 + b.nm[j + 1];
+            w0 = 1.0f / m0;
+            k_air = 0.6f * length * width;
+            limit = 0.5f * bone_mass * C::inv_h;
+            topo = topo_word(i0, C::trash, C::trash, C::trash);
+            const uint32_t pj = r[2];
+            if (j > 0 && pj != EVO_NONE) {
+                const uint32_t hub = i0, parent_pivot = bone_rec[pj * EVO_BONE_WORDS];
+                const bool at_head = hub == parent_pivot;
+                const uint32_t far_ = at_head ? pj + 1u : parent_pivot;
+                topo = topo_word(hub, pj + 1u, parent_pivot, far_) | (at_head ? kAtHead : 0u) | kHasJoint;
+                lo = bits_f(r[3]);
+                hi = bits_f(r[4]);
+                wf = 1.0f / bits_f(rec[far_ * EVO_NODE_WORDS]);
+            }
         }
+        b.btopo[j] = topo;
+        b.blen[j] = length;
+        b.bw0[j] = w0;
+        b.bk[j] = k_air;
+        b.blim[j] = limit;
+        b.br0[j] = r0;
+        b.jlo[j] = lo;
+        b.jhi[j] = hi;
+        b.jwf[j] = wf;
     });
-    loop<C, C::MAXM, 0>(b.muscles, [&](auto kc_) {
-        const int k = kc_;
-        const float4* q = reinterpret_cast<const float4*>(muscles + tr.mus_off + k * EVO_MUSCLE_WORDS);
-        const float4 r0 = q[0], r1 = q[1];
-        b.mnodes[k] = f_bits(r0.x);
-        b.maa[k] = r0.y;
-        b.mab[k] = r0.z;
-        b.mhill[k] = r0.w;
-        b.mcap[k] = r1.x;
-        b.minvcap[k] = r1.y;
-        b.mtk[k] = r1.z;
-        b.mslack[k] = r1.w;
+    all_items<C, C::MAXM>([&](auto kc) {
+        const int k = kc;
+        if (static_cast<uint32_t>(k) < b.muscles) {
+            const float4* q = reinterpret_cast<const float4*>(muscles + tr.mus_off + k * EVO_MUSCLE_WORDS);
+            const float4 r0 = q[0], r1 = q[1];
+            b.mnodes[k] = f_bits(r0.x);
+            b.maa[k] = r0.y;
+            b.mab[k] = r0.z;
+            b.mhill[k] = r0.w;
+            b.mcap[k] = r1.x;
+            b.minvcap[k] = r1.y;
+            if constexpr (C::tendon) {
+                b.mtk[k] = r1.z;
+                b.mslack[k] = r1.w;
+            }
+            s.o(k) = 0.0f;
+        } else {
+            b.mnodes[k] = C::trash | (C::trash << 8u) | (C::trash << 16u) | (C::trash << 24u);
+            b.maa[k] = 0.0f;
+            b.mab[k] = 0.0f;
+            b.mhill[k] = 0.0f;
+            b.mcap[k] = 0.0f;
+            b.minvcap[k] = 0.0f;
+            if constexpr (C::tendon) {
+                b.mtk[k] = 0.0f;
+                b.mslack[k] = 0.0f;
+            }
+        }
         b.mstore[k] = 1.0f;
         b.mdemand[k] = 0.0f;
         if constexpr (C::record) { b.mpull[k] = 0.0f; }
-        s.o(k) = 0.0f;
     });
     tr.fitness = 0.0f;
     tr.ground_contact = 0.0f;
@@ -723,7 +844,8 @@ __device__ __forceinline__ void begin_creature(Body<C>& b, Trial& tr, const Cols
     tr.scoring = true;
     tr.step = 0u;
     if constexpr (C::record) {
-        for (uint32_t t = 0u; t <= C::settle; t++) { record_frame<C>(b, s, frames, t, p.stride); }
+        const Bounds own{b.nodes, b.bones, b.muscles};
+        for (uint32_t t = 0u; t <= C::settle; t++) { record_frame<C>(b, s, frames, t, p.stride, own); }
     }
 }
 
@@ -760,20 +882,21 @@ __device__ __forceinline__ void store_result(evo_result* __restrict__ out, const
 }
 
 template <class C>
-__device__ __forceinline__ void step_metrics(const Body<C>& b, const Cols<C>& s, int I, const evo_params& p,
+__device__ __forceinline__ void step_metrics(const Body<C>& b, const Cols<C>& s, int i, const evo_params& p,
                                              uint32_t flags, bool& failed, float& center_y, float& com_x, float& low,
                                              float& high, uint32_t& touching, uint32_t& lifted) {
-    const float4 q = s.p(I);
+    if (static_cast<uint32_t>(i) >= b.nodes) { return; }
+    const float4 q = s.p(i);
     failed = failed || !(fabsf(q.x) <= 1e6f && fabsf(q.y) <= 1e6f);
     center_y += q.y;
-    com_x = __builtin_fmaf(b.nm[I], q.x, com_x);
-    low = fminf(low, q.y - b.nr[I]);
-    high = fmaxf(high, q.y + b.nr[I]);
+    com_x = __builtin_fmaf(b.nm[i], q.x, com_x);
+    low = fminf(low, q.y - b.nr[i]);
+    high = fmaxf(high, q.y + b.nr[i]);
     if (has<C, EVO_GROUND>(flags)) {
         const float2 g = ground<C>(q.x, b.amp, b.qphase, p, flags);
-        const float floor_y = __builtin_fmaf(b.nr[I], sqrtf(__builtin_fmaf(g.y, g.y, 1.0f)), g.x);
-        if (q.y <= floor_y + 0.002f) { touching |= 1u << I; }
-        if (q.y > floor_y + 0.01f) { lifted |= 1u << I; }
+        const float floor_y = __builtin_fmaf(b.nr[i], sqrtf(__builtin_fmaf(g.y, g.y, 1.0f)), g.x);
+        if (q.y <= floor_y + 0.002f) { touching |= 1u << i; }
+        if (q.y > floor_y + 0.01f) { lifted |= 1u << i; }
     }
 }
 
@@ -781,33 +904,31 @@ __device__ __forceinline__ void step_metrics(const Body<C>& b, const Cols<C>& s,
 template <class C>
 __device__ __forceinline__ void run_step(Body<C>& b, Trial& tr, const Cols<C>& s, const float* __restrict__ muscles,
                                          evo_result* __restrict__ results, const evo_params& p, uint32_t flags,
-                                         float rec_h, float2* __restrict__ frames) {
+                                         float rec_h, float2* __restrict__ frames, const Bounds& wb) {
     const uint32_t step = tr.step;
     const float* mus = muscles + tr.mus_off;
     const uint32_t cflags = tr.flags;
     const float t_now = static_cast<float>(step) * C::dt;
     if constexpr (C::record) {
-        loop<C, C::MAXN, 0>(static_cast<uint32_t>(C::MAXN), [&](auto ic_) {
-            const int i = ic_;
-            b.normal_force[i] = 0.0f;
-            b.friction_force[i] = 0.0f;
+        all_items<C, C::MAXN>([&](auto ic) {
+            b.normal_force[static_cast<int>(ic)] = 0.0f;
+            b.friction_force[static_cast<int>(ic)] = 0.0f;
         });
     }
-    muscle_demands<C>(b, s, mus, t_now, tr.limp);
-    for (uint32_t sub = 0u; sub < C::substeps; sub++) { substep<C>(b, s, p, flags, rec_h); }
+    muscle_demands<C>(b, s, mus, t_now, tr.limp, wb);
+    for (uint32_t sub = 0u; sub < C::substeps; sub++) { substep<C>(b, s, p, flags, rec_h, wb); }
     if constexpr (C::record) {
         const float inv_sub = 1.0f / static_cast<float>(C::substeps);
-        loop<C, C::MAXN, 0>(static_cast<uint32_t>(C::MAXN), [&](auto ic_) {
-            const int i = ic_;
-            b.normal_force[i] *= inv_sub;
-            b.friction_force[i] *= inv_sub;
+        all_items<C, C::MAXN>([&](auto ic) {
+            b.normal_force[static_cast<int>(ic)] *= inv_sub;
+            b.friction_force[static_cast<int>(ic)] *= inv_sub;
         });
     }
     bool failed = false;
     float center_y = 0.0f, com_x = 0.0f, low = 1e20f, high = -1e20f;
     uint32_t touching = 0u, lifted = 0u;
-    loop<C, C::MAXN, 0>(b.nodes, [&](auto ic_) {
-        step_metrics<C>(b, s, ic_, p, flags, failed, center_y, com_x, low, high, touching, lifted);
+    items<C, C::MAXN, 0, 1>(wb.n, [&](auto ic) {
+        step_metrics<C>(b, s, ic, p, flags, failed, center_y, com_x, low, high, touching, lifted);
     });
     center_y /= static_cast<float>(b.nodes);
     com_x *= b.inv_mass;
@@ -817,15 +938,14 @@ __device__ __forceinline__ void run_step(Body<C>& b, Trial& tr, const Cols<C>& s
     tr.ground_bits = touching;
     if (down != 0u && step > 0u) {
         const float next = t_now + C::dt;
-        loop<C, C::MAXM, 0>(b.muscles, [&](auto kc_) {
-            const int k = kc_;
+        for (uint32_t k = 0; k < b.muscles; k++) {
             const float4* q = reinterpret_cast<const float4*>(mus + k * EVO_MUSCLE_WORDS);
             const uint32_t sensor = f_bits(q[4].x);
             if (sensor != EVO_NONE && sensor < 32u && ((down >> sensor) & 1u)) {
                 const float4 r2 = q[2], r3 = q[3];
                 s.o(k) = fracf(r3.w - __builtin_fmaf(next, r2.y, r2.z));
             }
-        });
+        }
     }
     if ((step + 1u) % C::head_sample == 0u) {
         const float4 head = s.p(0);
@@ -841,7 +961,7 @@ __device__ __forceinline__ void run_step(Body<C>& b, Trial& tr, const Cols<C>& s
         tr.head_vel_x = vx;
         tr.head_vel_y = vy;
     }
-    const bool broken = broken_joints<C>(b, s) != 0u;
+    const bool broken = broken_joints<C>(b, s, wb) != 0u;
     if (tr.scoring) {
         tr.contact_lo = tr.contact_bits;
         tr.lift_lo = tr.lift_bits;
@@ -899,9 +1019,8 @@ __device__ __forceinline__ void run_step(Body<C>& b, Trial& tr, const Cols<C>& s
             const float speed = (com_x - tr.rung_x) * (C::rate / static_cast<float>(C::half_s));
             const float touched = static_cast<float>(__builtin_popcount(tr.contact_bits)) / static_cast<float>(b.nodes);
             float en_mean = 0.0f;
-            loop<C, C::MAXM, 0>(b.muscles, [&](auto kc_) {
-                const int k = kc_;
-                en_mean += b.mstore[k];
+            items<C, C::MAXM, 0, 1>(wb.m, [&](auto kc) {
+                if (static_cast<uint32_t>(static_cast<int>(kc)) < b.muscles) { en_mean += b.mstore[static_cast<int>(kc)]; }
             });
             en_mean /= static_cast<float>(b.muscles > 1u ? b.muscles : 1u);
             tr.contact_hi = (tr.contact_hi & keep) | (f2h(com_x) << sh);
@@ -921,11 +1040,11 @@ __device__ __forceinline__ void run_step(Body<C>& b, Trial& tr, const Cols<C>& s
                                      h2f(f2h(tr.head_shake)), h2f(f2h(en_mean)), tr.period_f};
                 float score = 0.0f;
                 bool finite = true;
-                unroll<6>([&](auto qc_) {
-                    constexpr int q = decltype(qc_)::value;
+#pragma unroll
+                for (int q = 0; q < 6; q++) {
                     score = __builtin_fmaf(rp.w[q], fv[q], score);
                     finite = finite && __builtin_isfinite(fv[q]);
-                });
+                }
                 if (finite && score < rp.bias && ((rp.off >> band) & 1u) == 0u) {
                     tr.screened = t_now + C::dt;
                     tr.screen_x = com_x;
@@ -990,7 +1109,7 @@ __device__ __forceinline__ void run_step(Body<C>& b, Trial& tr, const Cols<C>& s
         }
     }
     if constexpr (C::record) {
-        record_frame<C>(b, s, frames, C::settle + step + 1u, p.stride);
+        record_frame<C>(b, s, frames, C::settle + step + 1u, p.stride, wb);
         if (step + 1u >= p.steps) {
             store_result(results + tr.cidx, tr.kept);
             tr.live = false;
@@ -998,6 +1117,13 @@ __device__ __forceinline__ void run_step(Body<C>& b, Trial& tr, const Cols<C>& s
         }
     }
     tr.step = step + 1u;
+}
+
+// The largest of v over the wave's 64 lanes, as a scalar.
+__device__ __forceinline__ uint32_t wave_max(uint32_t v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) { v = max(v, static_cast<uint32_t>(__shfl_xor(static_cast<int>(v), off))); }
+    return __builtin_amdgcn_readfirstlane(v);
 }
 
 // Most creatures a wave claims from the launch's counter at a time, and the
@@ -1016,8 +1142,8 @@ __global__ void __launch_bounds__(C::BLOCK) advance_kernel(const uint32_t* __res
                                                           uint32_t* __restrict__ counter, const evo_params p,
                                                           uint32_t flags, float2* __restrict__ frames,
                                                           uint64_t frames_per_creature) {
-    __shared__ float4 s_pv[C::MAXN * C::BLOCK];
-    __shared__ float2 s_fp[C::MAXN * C::BLOCK];
+    __shared__ float4 s_pv[(C::MAXN + 1) * C::BLOCK];
+    __shared__ float2 s_fp[(C::MAXN + 1) * C::BLOCK];
     __shared__ float s_off[(C::MAXM > 0 ? C::MAXM : 1) * C::BLOCK];
     const uint32_t lane = threadIdx.x & 63u;
     const uint64_t below = (1ull << lane) - 1ull;
@@ -1030,6 +1156,7 @@ __global__ void __launch_bounds__(C::BLOCK) advance_kernel(const uint32_t* __res
     Body<C> b;
     Trial tr;
     tr.live = false;
+    b.nodes = b.bones = b.muscles = 0u;
     uint32_t win_cur = 0u, win_end = 0u;
     bool claimed_all = false;
     float2* my_frames = nullptr;
@@ -1075,7 +1202,9 @@ __global__ void __launch_bounds__(C::BLOCK) advance_kernel(const uint32_t* __res
             begin_creature<C>(b, tr, s, p.base + mine_at, records, muscles, heads, p, flags, my_frames);
         }
         if (__ballot(tr.live) == 0ull) { break; }
-        if (tr.live) { run_step<C>(b, tr, s, muscles, results, p, flags, rec_h, my_frames); }
+        const Bounds wb{wave_max(tr.live ? b.nodes : 0u), wave_max(tr.live ? b.bones : 0u),
+                        wave_max(tr.live ? b.muscles : 0u)};
+        if (tr.live) { run_step<C>(b, tr, s, muscles, results, p, flags, rec_h, my_frames, wb); }
     }
 }
 

@@ -135,8 +135,10 @@ typedef struct evo_result {
 
 /* How a batch runs: its world effect bits, steps per second (60 standard,
  * 120 fine), substeps per step (8 or 16), whether every frame of every
- * creature is recorded, and the largest node and muscle counts in the batch
- * (0 when unknown), which pick the kernel's body-size class. */
+ * creature is recorded, the largest node and muscle counts in the batch
+ * (0 when unknown), which pick the kernel's body-size class, and
+ * no_tendons = 1 when the caller knows that every muscle's tendon stiffness
+ * is 0 (which lets a kernel without tendon arithmetic run it). */
 typedef struct evo_run {
     uint32_t flags;
     uint32_t rate;
@@ -144,6 +146,7 @@ typedef struct evo_run {
     uint32_t record;
     uint32_t max_nodes;
     uint32_t max_muscles;
+    uint32_t no_tendons;
 } evo_run;
 
 #ifdef __cplusplus

@@ -32,6 +32,8 @@ def load() -> C.CDLL:
             fn = getattr(lib, name)
             fn.argtypes = [C.c_void_p] * 7 + [C.c_int, C.POINTER(C.c_long)]
             fn.restype = C.c_int
+        lib.oracle_count_ops.argtypes = [C.c_void_p] * 6 + [C.POINTER(C.c_long)]
+        lib.oracle_count_ops.restype = C.c_ulonglong
         lib.oracle_cos_0_pi_n.argtypes = [C.c_void_p, C.c_void_p, C.c_long]
         lib.oracle_atan2_n.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_long]
         lib.oracle_f2h_n.argtypes = [C.c_void_p, C.c_void_p, C.c_long]
@@ -61,6 +63,20 @@ def run(batch, params, run_, precision: str = "f32", threads: int = 1, frames: b
     if rc != 0:
         raise ValueError(f"oracle cannot run creature {-rc - 1}")
     return results, fr, steps.value
+
+
+def count_ops(batch, params, run_):
+    """Floating-point operations of the trials of creatures
+    [params.base, params.base + params.count), by the convention of
+    sim_count.cpp, and the steps simulated."""
+    from evolution_simulator_gpu_amd import _native as N
+
+    lib = load()
+    results = np.zeros(batch.count, dtype=N.RESULT_DTYPE)
+    steps = C.c_long(0)
+    ops = lib.oracle_count_ops(batch.records.ctypes.data, batch.muscles.ctypes.data, batch.heads.ctypes.data,
+                               C.addressof(params), C.addressof(run_), results.ctypes.data, C.byref(steps))
+    return int(ops), steps.value
 
 
 def run_f32(batch, params, run_, threads: int = 1, frames: bool = False):

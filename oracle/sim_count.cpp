@@ -1,0 +1,115 @@
+// ORACLE — test infrastructure only (see oracle_math.h).
+//
+// The oracle compiled with a counting scalar: every floating-point operation
+// of the trial adds its cost to a counter, so the algorithmic work per
+// creature-substep of a workload is measured on the same code the GPU is
+// checked against. Convention: add, subtract, multiply, divide, min, max,
+// floor, copysign and compare cost 1; a fused multiply-add costs 2; a square
+// root costs 1; negation and absolute value are free (operand modifiers);
+// cos on [0, pi] costs 14 and atan2 costs 32, the operations of the
+// routines in oracle_math.h. An operation whose operands are all launch
+// constants (literals and world settings) costs nothing: the kernel folds it
+// or computes it once per launch in scalar registers.
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <initializer_list>
+#include <type_traits>
+
+static thread_local unsigned long long g_ops = 0;
+
+struct CF {
+    double v;
+    bool k;  // a launch constant
+    CF() : v(0.0), k(true) {}
+    template <class T, class = typename std::enable_if<std::is_arithmetic<T>::value>::type>
+    CF(T x) : v(static_cast<double>(static_cast<float>(x))), k(false) {}
+    static CF lit(double x) { CF c; c.v = x; c.k = true; return c; }
+    explicit operator float() const { return static_cast<float>(v); }
+    explicit operator double() const { return v; }
+    explicit operator uint32_t() const { return static_cast<uint32_t>(v); }
+};
+
+static inline CF mk(double v, bool k) { CF c; c.v = v; c.k = k; return c; }
+static inline CF op1(double v, const CF& a, unsigned cost) { if (!a.k) { g_ops += cost; } return mk(v, a.k); }
+static inline CF op2(double v, const CF& a, const CF& b, unsigned cost) {
+    const bool k = a.k && b.k;
+    if (!k) { g_ops += cost; }
+    return mk(v, k);
+}
+static inline bool cmp(bool r, const CF& a, const CF& b) { if (!(a.k && b.k)) { g_ops += 1; } return r; }
+
+static inline CF operator+(const CF& a, const CF& b) { return op2(a.v + b.v, a, b, 1); }
+static inline CF operator-(const CF& a, const CF& b) { return op2(a.v - b.v, a, b, 1); }
+static inline CF operator*(const CF& a, const CF& b) { return op2(a.v * b.v, a, b, 1); }
+static inline CF operator/(const CF& a, const CF& b) { return op2(a.v / b.v, a, b, 1); }
+static inline CF operator-(const CF& a) { return mk(-a.v, a.k); }
+static inline CF& operator+=(CF& a, const CF& b) { a = a + b; return a; }
+static inline CF& operator-=(CF& a, const CF& b) { a = a - b; return a; }
+static inline CF& operator*=(CF& a, const CF& b) { a = a * b; return a; }
+static inline CF& operator/=(CF& a, const CF& b) { a = a / b; return a; }
+static inline bool operator<(const CF& a, const CF& b) { return cmp(a.v < b.v, a, b); }
+static inline bool operator>(const CF& a, const CF& b) { return cmp(a.v > b.v, a, b); }
+static inline bool operator<=(const CF& a, const CF& b) { return cmp(a.v <= b.v, a, b); }
+static inline bool operator>=(const CF& a, const CF& b) { return cmp(a.v >= b.v, a, b); }
+static inline bool operator==(const CF& a, const CF& b) { return cmp(a.v == b.v, a, b); }
+static inline bool operator!=(const CF& a, const CF& b) { return cmp(a.v != b.v, a, b); }
+
+static inline CF c_fma(const CF& a, const CF& b, const CF& c) {
+    const bool k = a.k && b.k && c.k;
+    if (!k) { g_ops += 2; }
+    return mk(std::fma(a.v, b.v, c.v), k);
+}
+static inline CF c_sqrt(const CF& a) { return op1(std::sqrt(a.v), a, 1); }
+static inline CF c_floor(const CF& a) { return op1(std::floor(a.v), a, 1); }
+static inline CF c_abs(const CF& a) { return mk(std::fabs(a.v), a.k); }
+static inline CF c_min(const CF& a, const CF& b) { return op2(std::fmin(a.v, b.v), a, b, 1); }
+static inline CF c_max(const CF& a, const CF& b) { return op2(std::fmax(a.v, b.v), a, b, 1); }
+static inline CF c_copysign(const CF& a, const CF& b) { return op2(std::copysign(a.v, b.v), a, b, 1); }
+static inline CF c_cos(const CF& a) { return op1(std::cos(a.v), a, 14); }
+static inline CF c_atan2(const CF& y, const CF& x) { return op2(std::atan2(y.v, x.v), y, x, 32); }
+
+#include "oracle_math.h"
+
+#define R CF
+#define FN(name) cnt_##name
+#define FMA c_fma
+#define SQRT c_sqrt
+#define FLOOR c_floor
+#define ABS c_abs
+#define MIN c_min
+#define MAX c_max
+#define COPYSIGN c_copysign
+#define COS_0_PI c_cos
+#define ATAN2 c_atan2
+#define LIT(x) CF::lit(x)
+#define SIM_FRAME_T float
+#define SIM_FRAME_BITS(p, lo, hi) do { (p)[0] = orc_as_float(lo); (p)[1] = orc_as_float(hi); } while (0)
+#pragma GCC diagnostic ignored "-Wclass-memaccess"
+#include "sim_impl.h"
+
+extern "C" {
+
+// Runs creatures [p->base, p->base + p->count) on one thread and returns the
+// operations counted; *steps_out receives the steps simulated.
+unsigned long long oracle_count_ops(const uint32_t* records, const float* muscles, const uint32_t* heads,
+                                    const evo_params* p, const evo_run* run, evo_result* results, long* steps_out) {
+    cnt_World w;
+    cnt_make_world(&w, p, run);
+    // World settings are launch constants.
+    for (CF* f : {&w.rate, &w.dt, &w.h, &w.inv_h, &w.gravity, &w.friction, &w.terrain, &w.muscle_recovery, &w.slope,
+                  &w.wind, &w.mud, &w.gaps, &w.hurdles, &w.quake, &w.screen_bar, &w.screen_bar_young,
+                  &w.screen_bar_reshaped, &w.water, &w.patches, &w.air_sub, &w.inv_muscle_energy, &w.brambles,
+                  &w.rec_h, &w.damp_share}) {
+        f->k = true;
+    }
+    g_ops = 0;
+    long steps = 0;
+    for (uint32_t c = 0; c < p->count; c++) {
+        const long s = cnt_run_creature(&w, p->base + c, records, muscles, heads, results + p->base + c, nullptr);
+        if (s > 0) { steps += s; }
+    }
+    if (steps_out) { *steps_out = steps; }
+    return g_ops;
+}
+}

@@ -13,7 +13,10 @@
  *     explicit FMA(a, b, c) = a*b + c rounded once;
  *   - sqrt and division are IEEE correctly rounded;
  *   - several quotients by one denominator are one reciprocal and products
- *     (muscle direction, joint gradients, ground normal);
+ *     (muscle direction, ground normal); the two joint gradients share one
+ *     reciprocal, 1 / (|u|^2 |v|^2);
+ *   - a muscle's pull is split into four rounded shares that are added to
+ *     the nodes' force pairs one by one, in node order a0, a1, b0, b1;
  *   - cos and atan2 are the routines of oracle_math.h (FP32) or libm (FP64).
  *
  * The including file defines R, FN(name), FMA, SQRT, FLOOR, ABS, MIN, MAX,
@@ -179,7 +182,9 @@ static inline void FN(joint)(const FN(Body)* b, uint32_t j, int with_angle, FN(J
         const R mid = b->jmid[j];
         k->angle = mid + FN(wrapr)(turn - mid);
     }
-    const R ivv = LIT(1.0) / vv, iuu = LIT(1.0) / uu;
+    /* One reciprocal serves both gradients. */
+    const R inv = LIT(1.0) / (uu * vv);
+    const R ivv = uu * inv, iuu = vv * inv;
     k->gtx = -vy * ivv;
     k->gty = vx * ivv;
     const R gux = -uy * iuu, guy = ux * iuu;
@@ -255,14 +260,16 @@ static void FN(muscle_forces)(const FN(World)* w, FN(Body)* b) {
         if (w->record) { b->mpull[k] = pull; }
         const R fx = ex * pull, fy = ey * pull;
         const R ra = LIT(1.0) - aa, rb = LIT(1.0) - ab;
-        b->fx[a0] = FMA(fx, ra, b->fx[a0]);
-        b->fy[a0] = FMA(fy, ra, b->fy[a0]);
-        b->fx[a1] = FMA(fx, aa, b->fx[a1]);
-        b->fy[a1] = FMA(fy, aa, b->fy[a1]);
-        b->fx[b0] = FMA(-fx, rb, b->fx[b0]);
-        b->fy[b0] = FMA(-fy, rb, b->fy[b0]);
-        b->fx[b1] = FMA(-fx, ab, b->fx[b1]);
-        b->fy[b1] = FMA(-fy, ab, b->fy[b1]);
+        /* Each share is rounded, then added: the HIP kernel adds them with
+         * in-order LDS float atomics. */
+        b->fx[a0] += fx * ra;
+        b->fy[a0] += fy * ra;
+        b->fx[a1] += fx * aa;
+        b->fy[a1] += fy * aa;
+        b->fx[b0] += -(fx * rb);
+        b->fy[b0] += -(fy * rb);
+        b->fx[b1] += -(fx * ab);
+        b->fy[b1] += -(fy * ab);
     }
 }
 
