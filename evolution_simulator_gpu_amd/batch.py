@@ -172,10 +172,10 @@ class DeviceBatch:
         fpc = N.lib().evo_settle_frames(rate) + p.steps + 1
         self.frames = torch.zeros(self.batch.count * fpc * p.stride * 2, dtype=torch.float32, device=self.device)
 
-    def launch(self, p: N.Params, run: N.Run, stream=None, split: bool = True) -> None:
-        """Simulates creatures [p.base, p.base + p.count) on `stream`, one
-        launch per body-size class when `split` and the range is the whole
-        batch."""
+    def launch(self, p: N.Params, run: N.Run, stream=None, split: bool = False) -> None:
+        """Simulates creatures [p.base, p.base + p.count) on `stream` in one
+        launch, or one launch per body-size class with `split` (slower: every
+        launch drains its own tail)."""
         import torch
 
         if stream is None:

@@ -1196,10 +1196,14 @@ __global__ void __launch_bounds__(C::BLOCK) advance_kernel(const uint32_t* __res
         }
         if (fresh) {
             tr.live = true;
+            // Claims run from the end of the range: a batch packed in
+            // ascending body size hands out its heaviest bodies first, and
+            // the light ones fill the tail of the launch.
+            const uint32_t at = p.count - 1u - mine_at;
             if constexpr (C::record) {
-                my_frames = frames + static_cast<uint64_t>(mine_at) * frames_per_creature * p.stride;
+                my_frames = frames + static_cast<uint64_t>(at) * frames_per_creature * p.stride;
             }
-            begin_creature<C>(b, tr, s, p.base + mine_at, records, muscles, heads, p, flags, my_frames);
+            begin_creature<C>(b, tr, s, p.base + at, records, muscles, heads, p, flags, my_frames);
         }
         if (__ballot(tr.live) == 0ull) { break; }
         const Bounds wb{wave_max(tr.live ? b.nodes : 0u), wave_max(tr.live ? b.bones : 0u),

@@ -1,0 +1,147 @@
+"""The C ABI without a GPU: every declared symbol is exported, the struct
+layouts agree, and the host-side logic (batch validation, world flags,
+synthetic batches) behaves as the reference's packer and world setup do.
+
+Host checks follow src/kernel.rs:477-486 (body size limits) and
+src/kernel.rs:82-113 (one world flag per active effect).
+"""
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import evolution_simulator_gpu_amd as E
+from evolution_simulator_gpu_amd import _native as N
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def _declared() -> set[str]:
+    text = "".join(p.read_text() for p in sorted((ROOT / "include").glob("*.h")))
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    text = re.sub(r"//[^\n]*", "", text)
+    return set(re.findall(r"\b(evo_[a-z0-9_]+)\s*\(", text))
+
+
+def test_library_exports_every_declared_symbol():
+    names = _declared()
+    assert len(names) >= 13
+    raw = C.CDLL(str(N.LIB_PATH))
+    missing = [n for n in sorted(names) if not hasattr(raw, n)]
+    assert not missing, f"declared in include/ but not exported: {missing}"
+    # and the ctypes mirror binds exactly the declared set
+    assert set(N.SIGNATURES) == names
+
+
+def test_struct_layouts_match_c():
+    N.check_layouts()
+    assert N.RESULT_DTYPE.itemsize == C.sizeof(N.Result)
+    info = N.lib().evo_build_info().decode()
+    assert "gfx950" in info
+
+
+def _mutate(batch, fn):
+    b = E.Batch(batch.records.copy(), batch.muscles.copy(), batch.heads.copy(), batch.slots.copy())
+    fn(b)
+    return b
+
+
+def _first_bad(batch):
+    bad = C.c_uint32(0xFFFFFFFF)
+    rc = N.lib().evo_validate_batch(N.ptr(batch.records), batch.records.size, N.ptr(batch.muscles),
+                                    batch.muscles.size, N.ptr(batch.heads), batch.count, C.byref(bad))
+    return rc, bad.value
+
+
+def test_validate_accepts_generated_batches():
+    for preset in range(4):
+        b = E.synth(E.synth_spec(preset, 300))
+        assert _first_bad(b)[0] == N.EVO_OK
+
+
+def _node_word(b, c, node, word):
+    return int(b.heads[c, 3]) + node * N.NODE_WORDS + word
+
+
+def _bone_word(b, c, bone, word):
+    n = int(b.heads[c, 0] & 255)
+    return int(b.heads[c, 3]) + n * N.NODE_WORDS + bone * N.BONE_WORDS + word
+
+
+def _muscle_word(b, c, k, word):
+    return int(b.heads[c, 4]) + k * N.MUSCLE_WORDS + word
+
+
+BAD_EDITS = {
+    "one node": lambda b, c: b.heads.__setitem__((c, 0), (int(b.heads[c, 0]) & 0xFFFFFF00) | 1),
+    "33 nodes": lambda b, c: b.heads.__setitem__((c, 0), (int(b.heads[c, 0]) & 0xFFFFFF00) | 33),
+    "97 muscles": lambda b, c: b.heads.__setitem__((c, 1), 97),
+    "records beyond the buffer": lambda b, c: b.heads.__setitem__((c, 3), b.records.size - 3),
+    "muscles beyond the buffer": lambda b, c: b.heads.__setitem__((c, 4), b.muscles.size - 4),
+    "unaligned muscle offset": lambda b, c: b.heads.__setitem__((c, 4), b.heads[c, 4] + 1),
+    "zero mass": lambda b, c: b.records.__setitem__(_node_word(b, c, 0, 0), 0),
+    "negative radius": lambda b, c: b.records.__setitem__(_node_word(b, c, 1, 1),
+                                                          np.float32(-0.1).view(np.uint32)),
+    "pivot after its bone": lambda b, c: b.records.__setitem__(_bone_word(b, c, 1, 0), 5),
+    "parent after its bone": lambda b, c: b.records.__setitem__(_bone_word(b, c, 2, 2), 2),
+    "muscle node out of range": lambda b, c: b.muscles.view(np.uint32).__setitem__(
+        _muscle_word(b, c, 0, 0), 0x00010240),
+    "sensor out of range": lambda b, c: b.muscles.view(np.uint32).__setitem__(_muscle_word(b, c, 0, 16), 40),
+}
+
+
+@pytest.mark.parametrize("name", sorted(BAD_EDITS))
+def test_validate_names_the_first_bad_creature(name):
+    b = E.synth(E.synth_spec(1, 64))
+    nodes, muscles = b.nodes(), b.muscle_counts()
+    c = int(np.nonzero((nodes >= 6) & (muscles >= 1))[0][5])
+    bad = _mutate(b, lambda x: BAD_EDITS[name](x, c))
+    rc, first = _first_bad(bad)
+    assert rc == N.EVO_ERR_INVALID and first == c
+    with pytest.raises(ValueError):
+        bad.validate()
+
+
+def test_max_muscles_over_a_range():
+    b = E.synth(E.synth_spec(1, 500))
+    m = b.muscle_counts()
+    lib = N.lib()
+    for base, count in [(0, 500), (10, 7), (499, 1), (0, 0)]:
+        assert lib.evo_max_muscles(N.ptr(b.heads), base, count) == int(m[base:base + count].max(initial=0))
+
+
+def test_world_flags_name_each_active_effect():
+    lib = N.lib()
+    w = E.world()
+    assert lib.evo_world_flags(C.byref(w)) == N.GROUND
+    effects = {"terrain": (0.1, N.TERRAIN), "slope": (0.1, N.SLOPE), "gaps": (0.3, N.GAPS),
+               "hurdles": (0.1, N.HURDLES), "quake": (0.05, N.QUAKE | N.TERRAIN), "mud": (0.05, N.MUD),
+               "water": (0.3, N.WATER), "patches": (0.5, N.ICE), "wind": (-2.0, N.WIND),
+               "air_retention": (0.98, N.AIR), "brambles": (5.0, N.BRAMBLES)}
+    for name, (value, bit) in effects.items():
+        w = E.world(**{name: value})
+        assert lib.evo_world_flags(C.byref(w)) == N.GROUND | bit, name
+    # without ground only water, wind and air remain (src/kernel.rs:90-104)
+    w = E.world(ground=0, terrain=0.1, slope=0.1, gaps=0.3, mud=0.05, patches=0.5, water=0.3, wind=1.0,
+                air_retention=0.98, brambles=5.0, quake=0.05, hurdles=0.1)
+    assert lib.evo_world_flags(C.byref(w)) == N.WATER | N.WIND | N.AIR
+
+
+def test_synthetic_batches_are_deterministic_and_sorted():
+    a = E.synth(E.synth_spec(1, 2000))
+    b = E.synth(E.synth_spec(1, 2000))
+    assert np.array_equal(a.records, b.records) and np.array_equal(a.muscles.view(np.uint32),
+                                                                   b.muscles.view(np.uint32))
+    assert np.array_equal(a.heads, b.heads)
+    c = E.synth(E.synth_spec(1, 2000, seed=99))
+    assert not np.array_equal(a.heads[:, 6:8], c.heads[:, 6:8]) or not np.array_equal(a.records, c.records)
+    key = a.muscle_counts().astype(np.int64) * 64 + a.nodes()
+    assert np.all(np.diff(key) >= 0)
+    # slots are a permutation: every caller index appears once
+    assert np.array_equal(np.sort(a.slots), np.arange(a.count, dtype=np.uint32))
+    # presets stay within their stated limits
+    for preset, (lo, hi, mmax) in {0: (3, 6, 8), 1: (3, 12, 24), 2: (3, 32, 64)}.items():
+        s = E.synth(E.synth_spec(preset, 400))
+        assert s.nodes().min() >= lo and s.nodes().max() <= hi and s.muscle_counts().max() <= mmax
