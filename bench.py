@@ -28,11 +28,16 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 # Algorithmic FP32 operations per creature-substep of this workload, counted
-# by the oracle compiled with a counting scalar (oracle/sim_count.cpp) on a
-# 4,096-creature sample of the same workload; tests/test_flops.py re-measures
-# it. Convention: add/sub/mul/div/min/max/floor/compare = 1, FMA = 2,
-# sqrt = 1, cos = 14, atan2 = 32.
-FLOPS_PER_CREATURE_SUBSTEP = 2107.6
+# by the oracle compiled with a counting scalar (oracle/sim_count.cpp) over an
+# 8,192-creature sample strided over the same 3,000,000-creature batch (20 s
+# trials); tests/test_flops.py re-measures it on a smaller sample. Convention:
+# add/sub/mul/div/min/max/floor/compare = 1, FMA = 2, sqrt = 1, cos = 14,
+# atan2 = 32; operations on launch constants are free.
+FLOPS_PER_CREATURE_SUBSTEP = 2148.7
+# Bytes the forward launch of this workload moves beyond L2, from rocprofv3
+# FETCH_SIZE and WRITE_SIZE scaled by a 256 MiB copy (reads x 2.0, writes x 1.0);
+# profiles/advance_kernel_plain_forward_3M.md gives the runs.
+TRAFFIC_BYTES_PER_LAUNCH = 1.578e12
 FP32_PEAK_TFLOPS = 157.3  # MI355X dense FP32, 256 CUs at 2.4 GHz
 
 
@@ -43,9 +48,12 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--creatures", type=int, default=3_000_000)
     ap.add_argument("--duration", type=float, default=20.0)
-    ap.add_argument("--cpu-sample", type=int, default=16384)
+    ap.add_argument("--cpu-sample", type=int, default=98304)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--replays", type=int, default=64)
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="only run 3 device-to-device copies of this many MiB (the scale of the HBM byte counters)")
     return ap.parse_args()
 
 
@@ -76,6 +84,16 @@ def main():
         sys.exit(2)
     import torch
 
+    if args.copy_mib:
+        n = args.copy_mib << 18
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "copies": 3}))
+        return
+
     import evolution_simulator_gpu_amd as E
     from evolution_simulator_gpu_amd import _native as N
 
@@ -90,23 +108,40 @@ def main():
     torch.cuda.synchronize()
     stream = torch.cuda.current_stream()
 
-    for _ in range(args.warmup):
+    def step(marks=None):
+        # one generation: the forward trials, the archive metrics and niche
+        # cells of every creature, and the replay capture of the fittest
         db.launch(p, run, stream)
+        if marks is not None:
+            marks[0].record(stream)
+        db.describe(p.steps, world.rate, N.LAYOUT_GLOBAL, stream, out=metrics)
+        if marks is not None:
+            marks[1].record(stream)
+        return db.capture_top(args.replays, p, run, stream)
+
+    metrics = torch.empty(batch.count * N.METRICS_DTYPE.itemsize, dtype=torch.uint8, device="cuda")
+    for _ in range(args.warmup):
+        step()
     torch.cuda.synchronize()
 
-    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ev = lambda: torch.cuda.Event(enable_timing=True)  # noqa: E731
+    marks = [[ev(), ev(), ev(), ev()] for _ in range(args.steps)]
     torch.cuda.synchronize()
     t_begin = time.perf_counter()
     for i in range(args.steps):
-        starts[i].record(stream)
-        db.launch(p, run, stream)
-        ends[i].record(stream)
+        marks[i][0].record(stream)
+        replay = step(marks[i][1:3])
+        marks[i][3].record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t_begin
-    kernel_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
+    kernel_ms = [m[0].elapsed_time(m[1]) for m in marks]
+    describe_ms = [m[1].elapsed_time(m[2]) for m in marks]
+    replay_ms = [m[2].elapsed_time(m[3]) for m in marks]
 
     res = db.results_numpy()
+    pick, _, replay_res = replay
+    same = np.array_equal(replay_res.cpu().numpy().view(np.uint32),
+                          res.view(np.float32).reshape(-1, 19)[pick.cpu().numpy()].view(np.uint32).reshape(-1))
     steps_run = (res["ground_hi"].view(np.uint32) >> 16).astype(np.int64)
     substeps_per_gen = int(steps_run.sum()) * world.substeps
     value = substeps_per_gen * args.steps / wall
@@ -134,12 +169,18 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": "plain forward: 3,000,000 creatures from seed 2, 3-12 nodes, bones + muscles <= 24, "
-                        "flat ground, 20 s trials at 60 Hz x 8 substeps",
+                        "flat ground, 20 s trials at 60 Hz x 8 substeps; archive metrics and niche cells of "
+                        "every creature; replay capture of the 64 fittest",
             "creatures": batch.count,
             "substeps_per_generation": substeps_per_gen,
             "fell_share": float((res["fall_time"] > 0).mean()),
             "generate_s": gen_s,
             "kernel": N.lib().evo_build_info().decode(),
+            "forward_ms": float(np.mean(kernel_ms)),
+            "describe_ms": float(np.mean(describe_ms)),
+            "replay_ms": float(np.mean(replay_ms)),
+            "replays": args.replays,
+            "replays_match_trials": same,
         },
         "roofline": {
             "bound": "valu",
@@ -147,7 +188,8 @@ def main():
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP32_PEAK_TFLOPS,
-            "traffic": None,
+            "traffic": TRAFFIC_BYTES_PER_LAUNCH if args.creatures == 3_000_000 and args.duration == 20.0 else None,
+            "traffic_unit": "B per launch (rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0)",
             "flops_per_creature_substep": FLOPS_PER_CREATURE_SUBSTEP,
             "avg_launch_ms": avg_launch_s * 1e3,
         },

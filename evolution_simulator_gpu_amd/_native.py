@@ -95,6 +95,17 @@ class SynthSpec(C.Structure):
 RESULT_DTYPE = np.dtype([(name, np.float32) for name in RESULT_FIELDS])
 assert RESULT_DTYPE.itemsize == 76
 
+# Niche layouts (src/qd.rs:56-84) and the archive metrics of one trial.
+LAYOUT_ISLAND = 0
+LAYOUT_GLOBAL = 1
+METRICS_DTYPE = np.dtype([
+    ("fitness", np.float32), ("ground_contact", np.float32), ("vertical_oscillation", np.float32),
+    ("gait_frequency", np.float32), ("mean_height", np.float32), ("feet", np.float32), ("screen_x", np.float32),
+    ("aspect_ratio", np.float32), ("steps", np.uint32), ("niche", np.uint8, (6,)), ("screened", np.uint8),
+    ("fell", np.uint8),
+])
+assert METRICS_DTYPE.itemsize == 44
+
 _P = C.c_void_p
 _u32 = C.c_uint32
 _u64 = C.c_uint64
@@ -111,6 +122,7 @@ SIGNATURES = {
     "evo_synth_sizes": ([C.POINTER(SynthSpec), C.POINTER(_u64), C.POINTER(_u64)], C.c_int),
     "evo_synth_fill": ([C.POINTER(SynthSpec), _P, _P, _P, _P, _P, _P], C.c_int),
     "evo_advance_device": ([_P, _P, _P, _P, _P, C.POINTER(Params), C.POINTER(Run), _P, _P], C.c_int),
+    "evo_describe_device": ([_P, _P, _P, _u32, _u32, _u32, _u32, _u32, _P, _P], C.c_int),
     "evo_validate_batch": ([_P, _u64, _P, _u64, _P, _u32, C.POINTER(_u32)], C.c_int),
     "evo_max_muscles": ([_P, _u32, _u32], _u32),
     "evo_build_info": ([], C.c_char_p),

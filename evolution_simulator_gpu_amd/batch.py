@@ -195,6 +195,54 @@ class DeviceBatch:
             if rc != N.EVO_OK:
                 raise RuntimeError(f"evo_advance_device failed: {rc}")
 
+    def describe(self, steps: int, rate: int, layout: int = N.LAYOUT_GLOBAL, stream=None, out=None):
+        """Archive metrics and niche cells of every creature from the last
+        results, on the device: a uint8 tensor of count x 44 bytes
+        (N.METRICS_DTYPE rows)."""
+        import torch
+
+        if stream is None:
+            stream = torch.cuda.current_stream(self.records.device)
+        if out is None:
+            out = torch.empty(self.batch.count * N.METRICS_DTYPE.itemsize, dtype=torch.uint8, device=self.device)
+        rc = N.lib().evo_describe_device(self.records.data_ptr(), self.heads.data_ptr(), self.results.data_ptr(), 0,
+                                         self.batch.count, steps, rate, layout, out.data_ptr(),
+                                         C.c_void_p(stream.cuda_stream))
+        if rc != N.EVO_OK:
+            raise RuntimeError(f"evo_describe_device failed: {rc}")
+        return out
+
+    def capture_top(self, k: int, p: N.Params, run: N.Run, stream=None):
+        """Replay capture: runs the k fittest creatures of the last results
+        again with every frame recorded (src/engine.rs:1032-1056 records one
+        creature per call; here one launch records all k). The k creatures
+        are picked on the device and reached through a gathered copy of their
+        head words, so no record moves. Returns (packed indices, frames,
+        results of the recorded trials) as device tensors; the frames of the
+        i-th pick start at i * (settle + steps + 1) * p.stride float pairs."""
+        import torch
+
+        if stream is None:
+            stream = torch.cuda.current_stream(self.records.device)
+        k = min(k, self.batch.count)
+        fit = self.results.view(-1, len(N.RESULT_FIELDS))[:, 0]
+        pick = torch.topk(torch.nan_to_num(fit, nan=float("-inf")), k).indices
+        heads = self.heads.view(-1, 8).index_select(0, pick).contiguous()
+        q = N.Params.from_buffer_copy(p)
+        q.base, q.count = 0, k
+        r = N.Run.from_buffer_copy(run)
+        r.record = 1
+        fpc = N.lib().evo_settle_frames(run.rate) + p.steps + 1
+        # slots past a creature's own nodes and muscles are not written: zero them
+        frames = torch.zeros(k * fpc * p.stride * 2, dtype=torch.float32, device=self.device)
+        results = torch.empty(k * len(N.RESULT_FIELDS), dtype=torch.float32, device=self.device)
+        rc = N.lib().evo_advance_device(self.records.data_ptr(), self.muscles.data_ptr(), heads.data_ptr(),
+                                        results.data_ptr(), self.counter.data_ptr() + 12, C.byref(q), C.byref(r),
+                                        frames.data_ptr(), C.c_void_p(stream.cuda_stream))
+        if rc != N.EVO_OK:
+            raise RuntimeError(f"evo_advance_device (recording) failed: {rc}")
+        return pick, frames, results
+
     def results_numpy(self) -> np.ndarray:
         return self.results.cpu().numpy().view(N.RESULT_DTYPE).reshape(-1)
 

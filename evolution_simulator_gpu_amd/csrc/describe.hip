@@ -1,0 +1,163 @@
+// Archive metrics and niche cells of finished trials, on the device.
+//
+// One lane per creature: it reads the creature's result (76 B), its head words
+// and the start positions of its node records, and writes one evo_metrics
+// (44 B). The work is a few dozen operations per creature, so the kernel is
+// bound by HBM traffic: about 76 + 32 + 8N bytes read and 44 written per
+// creature.
+//
+// Semantics follow the reference's host code, with Rust's float rules:
+// `clamp` keeps NaN, `max`/`min` drop it, `round` rounds half away from zero,
+// float-to-integer casts saturate (NaN to 0).
+//   trial_steps        src/scheduler.rs:859-870
+//   to_metrics         src/scheduler.rs:890-923
+//   descriptor         src/qd.rs:762-787
+//   niche_in           src/qd.rs:797-809, classes_of :125-131, bin :843-847
+// The height bin compares the mean height with the bin edges
+// 0.15 m * 8^(k/6) = 0.15 m * 2^(k/2) instead of evaluating the logarithm of
+// height_axis (src/qd.rs:837-842): the two are the same function of the
+// height, and the comparison needs no transcendental whose last bit would
+// depend on a math library.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../include/evo_engine.h"
+
+namespace {
+
+__device__ inline float rclamp(float x, float lo, float hi) { return x < lo ? lo : (x > hi ? hi : x); }
+
+__device__ inline uint32_t sat_u8(float x) {
+    if (!(x > 0.0f)) { return 0u; }
+    return x >= 255.0f ? 255u : static_cast<uint32_t>(x);
+}
+
+__device__ inline uint32_t sat_u32(float x) {
+    if (!(x > 0.0f)) { return 0u; }
+    return x >= 4294967296.0f ? 0xffffffffu : static_cast<uint32_t>(x);
+}
+
+__device__ inline int32_t sat_i32(float x) {
+    if (x != x) { return 0; }
+    if (x <= -2147483648.0f) { return INT32_MIN; }
+    return x >= 2147483648.0f ? INT32_MAX : static_cast<int32_t>(x);
+}
+
+// bin(value, low, high, count) of src/qd.rs:843-847.
+__device__ inline uint32_t bin(float v, float low, float high, uint32_t count) {
+    const float t = (rclamp(v, low, high) - low) / (high - low) * static_cast<float>(count);
+    const uint32_t b = sat_u8(floorf(t));
+    return b < count - 1u ? b : count - 1u;
+}
+
+// 0.15f * 2^(k/2) for k = 1..5, rounded to float: the edges of the six
+// height bins.
+__constant__ float kHeightEdge[5] = {0x1.b27248p-3f, 0x1.333334p-2f, 0x1.b27248p-2f, 0x1.333334p-1f,
+                                     0x1.b27248p-1f};
+__constant__ float kIslandAspect[2] = {1.2f, 2.0f};
+__constant__ uint32_t kIslandNodes[2] = {9u, 12u};
+__constant__ float kGlobalAspect[3] = {0.9f, 1.4f, 2.5f};
+__constant__ uint32_t kGlobalNodes[3] = {9u, 11u, 14u};
+
+__global__ void __launch_bounds__(256) describe_kernel(const uint32_t* __restrict__ records,
+                                                       const uint4* __restrict__ heads,
+                                                       const evo_result* __restrict__ results, uint32_t base,
+                                                       uint32_t count, uint32_t steps_full, float rate,
+                                                       uint32_t layout, evo_metrics* __restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) { return; }
+    const uint32_t c = base + i;
+    const uint4 h0 = heads[2u * c];
+    const evo_result r = results[c];
+    const uint32_t nodes = h0.x & 255u;
+
+    // trial_steps: to the fall, else the screen, else the end.
+    const float ended = r.fall_time > 0.0f ? r.fall_time : r.screened;
+    const uint32_t full = steps_full > 1u ? steps_full : 1u;
+    uint32_t steps = full;
+    if (ended > 0.0f) {
+        const uint32_t s = sat_u32(roundf(ended * rate));
+        steps = s < 1u ? 1u : (s > full ? full : s);
+    }
+
+    // to_metrics
+    const float contact_den = static_cast<float>(steps * nodes);
+    const float contact = rclamp(r.ground_contact / contact_den, 0.0f, 1.0f);
+    const float vosc = isfinite(r.vertical_oscillation) ? fmaxf(r.vertical_oscillation, 0.0f) : 0.0f;
+    const float gait = isfinite(r.gait_frequency) ? fmaxf(r.gait_frequency, 0.0f) : 0.0f;
+    const float height = fmaxf(r.height_sum / static_cast<float>(steps), 0.0f);
+    const float feet = static_cast<float>(__popc(__float_as_uint(r.lift_lo)));
+
+    // descriptor: the start pose's bounding box
+    const uint32_t* rec = records + h0.w;
+    float min_x = INFINITY, max_x = -INFINITY, min_y = INFINITY, max_y = -INFINITY;
+    for (uint32_t n = 0; n < nodes; n++) {
+        const float x = __uint_as_float(rec[n * EVO_NODE_WORDS + 3u]);
+        const float y = __uint_as_float(rec[n * EVO_NODE_WORDS + 4u]);
+        min_x = fminf(min_x, x);
+        max_x = fmaxf(max_x, x);
+        min_y = fminf(min_y, y);
+        max_y = fmaxf(max_y, y);
+    }
+    const float aspect = rclamp(fmaxf(max_x - min_x, 0.01f) / fmaxf(max_y - min_y, 0.01f), 0.0625f, 16.0f);
+    const float d_contact = rclamp(contact, 0.0f, 1.0f);
+    const float d_gait = rclamp(gait, 0.0f, 20.0f);
+    const float d_height = fmaxf(height, 0.0f);
+    const float d_feet = fmaxf(feet, 0.0f);
+
+    // niche_in
+    uint32_t shape = 0u, size = 0u;
+    if (layout == EVO_LAYOUT_GLOBAL) {
+        for (int k = 0; k < 3; k++) {
+            shape += aspect >= kGlobalAspect[k] ? 1u : 0u;
+            size += nodes >= kGlobalNodes[k] ? 1u : 0u;
+        }
+    } else {
+        for (int k = 0; k < 2; k++) {
+            shape += aspect >= kIslandAspect[k] ? 1u : 0u;
+            size += nodes >= kIslandNodes[k] ? 1u : 0u;
+        }
+    }
+    uint32_t hbin = 0u;
+    for (int k = 0; k < 5; k++) { hbin += d_height >= kHeightEdge[k] ? 1u : 0u; }
+    int32_t fb = sat_i32(roundf(d_feet));
+    fb = fb < 1 ? 1 : (fb > 5 ? 5 : fb);
+
+    evo_metrics m;
+    m.fitness = r.fitness;
+    m.ground_contact = contact;
+    m.vertical_oscillation = vosc;
+    m.gait_frequency = gait;
+    m.mean_height = height;
+    m.feet = feet;
+    m.screen_x = r.screen_x;
+    m.aspect_ratio = aspect;
+    m.steps = steps;
+    m.niche[0] = static_cast<uint8_t>(bin(d_contact, 0.0f, 1.0f, 6u));
+    m.niche[1] = static_cast<uint8_t>(bin(d_gait, 0.0f, 6.0f, 8u));
+    m.niche[2] = static_cast<uint8_t>(shape);
+    m.niche[3] = static_cast<uint8_t>(hbin);
+    m.niche[4] = static_cast<uint8_t>(fb - 1);
+    m.niche[5] = static_cast<uint8_t>(size);
+    m.screened = r.screened > 0.0f ? 1u : 0u;
+    m.fell = (__float_as_uint(r.ground_hi) & 16u) ? 1u : 0u;
+    out[i] = m;
+}
+
+}  // namespace
+
+extern "C" int evo_describe_device(const uint32_t* d_records, const uint32_t* d_heads, const evo_result* d_results,
+                                   uint32_t base, uint32_t count, uint32_t steps, uint32_t rate, uint32_t layout,
+                                   evo_metrics* d_out, void* stream) {
+    if (!d_records || !d_heads || !d_results || !d_out || rate == 0u) { return EVO_ERR_INVALID; }
+    if (layout != EVO_LAYOUT_ISLAND && layout != EVO_LAYOUT_GLOBAL) { return EVO_ERR_INVALID; }
+    if (count == 0u) { return EVO_OK; }
+    const uint32_t blocks = (count + 255u) / 256u;
+    hipLaunchKernelGGL(describe_kernel, dim3(blocks), dim3(256), 0, reinterpret_cast<hipStream_t>(stream), d_records,
+                       reinterpret_cast<const uint4*>(d_heads), d_results, base, count, steps,
+                       static_cast<float>(rate), layout, d_out);
+    const hipError_t e = hipGetLastError();
+    if (e == hipErrorOutOfMemory) { return EVO_ERR_OOM; }
+    return e == hipSuccess ? EVO_OK : EVO_ERR_DEVICE;
+}

@@ -80,6 +80,12 @@ Launcher pick_size(const evo_run& run) {
 
 Launcher EVO_PICK(const evo_run& run) {
     if (run.record) {
+        // replays of bodies that fit the unrolled mid class (the usual top-k
+        // capture) run unrolled; larger or unknown bodies run the looped class
+        if (run.max_nodes != 0u && run.max_nodes <= static_cast<uint32_t>(kMidN) &&
+            run.max_muscles <= static_cast<uint32_t>(kMidM)) {
+            return &launch<Cfg<EVO_RATE, EVO_SUB, kAnyWorld, kMidN, kMidM, true, 128, true, true>>;
+        }
         return &launch<Cfg<EVO_RATE, EVO_SUB, kAnyWorld, EVO_MAX_NODES, EVO_MAX_MUSCLES, true, 64, false, true>>;
     }
     return run.flags == kFlat ? pick_size<kFlat>(run) : pick_size<kAnyWorld>(run);

@@ -685,12 +685,18 @@ __device__ __forceinline__ void record_frame(const Body<C>& b, const Cols<C>& s,
                                              uint32_t t, uint32_t stride, const Bounds& wb) {
     if constexpr (C::record) {
         float2* f = frames + static_cast<size_t>(t) * stride;
-        for (uint32_t i = 0; i < b.nodes; i++) {
-            const float4 q = s.p(i);
-            f[i] = make_float2(q.x, q.y);
-            f[EVO_MAX_NODES + b.muscles + i] = make_float2(b.normal_force[i], b.friction_force[i]);
-        }
-        for (uint32_t k = 0; k < b.muscles; k++) { f[EVO_MAX_NODES + k] = make_float2(b.mstore[k], b.mpull[k]); }
+        all_items<C, C::MAXN>([&](auto ic) {
+            const int i = ic;
+            if (static_cast<uint32_t>(i) < b.nodes) {
+                const float4 q = s.p(i);
+                f[i] = make_float2(q.x, q.y);
+                f[EVO_MAX_NODES + b.muscles + i] = make_float2(b.normal_force[i], b.friction_force[i]);
+            }
+        });
+        all_items<C, C::MAXM>([&](auto kc) {
+            const int k = kc;
+            if (static_cast<uint32_t>(k) < b.muscles) { f[EVO_MAX_NODES + k] = make_float2(b.mstore[k], b.mpull[k]); }
+        });
         const uint64_t broken = broken_joints<C>(b, s, wb);
         f[stride - 1u] = make_float2(bits_f(static_cast<uint32_t>(broken)), bits_f(static_cast<uint32_t>(broken >> 32)));
     }

@@ -150,6 +150,18 @@ int evo_advance_device(const uint32_t* d_records, const float* d_muscles, const 
                        evo_result* d_results, uint32_t* d_counter, const evo_params* p, const evo_run* run,
                        float* d_frames, void* stream);
 
+/* Archive metrics and niche cells of creatures [base, base + count) from
+ * their results, on the device: scheduler::to_metrics and trial_steps
+ * (src/scheduler.rs:859-923), then qd::descriptor and Descriptor::niche_in
+ * (src/qd.rs:762-809, :837-847), written to d_out[c - base]. `steps` is the
+ * trial's full step count (Config::steps), `rate` its steps per second and
+ * `layout` EVO_LAYOUT_ISLAND or EVO_LAYOUT_GLOBAL. The start pose's aspect
+ * ratio is read from the node records' start positions. Asynchronous on
+ * `stream`. */
+int evo_describe_device(const uint32_t* d_records, const uint32_t* d_heads, const evo_result* d_results,
+                        uint32_t base, uint32_t count, uint32_t steps, uint32_t rate, uint32_t layout,
+                        evo_metrics* d_out, void* stream);
+
 /* Host check of a packed batch: counts within the simulator's limits
  * (2-32 nodes, 0-96 muscles), every node index in range, bone pivots before
  * their child, parents before their bone, offsets inside the buffers.

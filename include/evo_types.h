@@ -149,15 +149,42 @@ typedef struct evo_run {
     uint32_t no_tendons;
 } evo_run;
 
+/* Archive layouts of the niche bytes (src/qd.rs:56-84): ISLAND classes split
+ * shapes at aspect 1.2 and 2.0 and sizes at 9 and 12 nodes; GLOBAL classes at
+ * aspect 0.9, 1.4, 2.5 and at 9, 11, 14 nodes. */
+#define EVO_LAYOUT_ISLAND 0u
+#define EVO_LAYOUT_GLOBAL 1u
+
+/* What the archives read of one trial: scheduler::to_metrics
+ * (src/scheduler.rs:859-923) and qd::descriptor + Descriptor::niche_in
+ * (src/qd.rs:762-809). The floats are the TrialMetrics values; `niche` holds
+ * the cell bytes: contact, cadence, shape class, height, feet, size class. */
+typedef struct evo_metrics {
+    float fitness;
+    float ground_contact;
+    float vertical_oscillation;
+    float gait_frequency;
+    float mean_height;
+    float feet;
+    float screen_x;
+    float aspect_ratio;
+    uint32_t steps;    /* trial_steps: to the fall, the screen or the end */
+    uint8_t niche[6];
+    uint8_t screened;  /* 1 when the screen stopped the trial */
+    uint8_t fell;      /* 1 when the trial ended in a fall (end code bit 4) */
+} evo_metrics;
+
 #ifdef __cplusplus
 }
 static_assert(sizeof(evo_rung) == 32, "evo_rung layout");
 static_assert(sizeof(evo_params) == 164, "evo_params layout");
 static_assert(sizeof(evo_result) == 76, "evo_result layout");
+static_assert(sizeof(evo_metrics) == 44, "evo_metrics layout");
 #else
 _Static_assert(sizeof(evo_rung) == 32, "evo_rung layout");
 _Static_assert(sizeof(evo_params) == 164, "evo_params layout");
 _Static_assert(sizeof(evo_result) == 76, "evo_result layout");
+_Static_assert(sizeof(evo_metrics) == 44, "evo_metrics layout");
 #endif
 
 #endif /* EVO_TYPES_H */

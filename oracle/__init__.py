@@ -5,6 +5,6 @@ built as oracle/_build/liboracle.so by oracle/Makefile. Only tests/,
 __graft_entry__.smoke() and bench.py's CPU-baseline leg may use it, as the
 checker. The product never imports it.
 """
-from .oracle import build, count_ops, load, run, run_f32, run_f64
+from .oracle import build, count_ops, describe, load, niche_in, run, run_f32, run_f64
 
-__all__ = ["build", "count_ops", "load", "run", "run_f32", "run_f64"]
+__all__ = ["build", "count_ops", "describe", "load", "niche_in", "run", "run_f32", "run_f64"]

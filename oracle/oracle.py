@@ -34,6 +34,10 @@ def load() -> C.CDLL:
             fn.restype = C.c_int
         lib.oracle_count_ops.argtypes = [C.c_void_p] * 6 + [C.POINTER(C.c_long)]
         lib.oracle_count_ops.restype = C.c_ulonglong
+        lib.oracle_describe.argtypes = [C.c_void_p] * 3 + [C.c_uint32] * 5 + [C.c_void_p]
+        lib.oracle_describe.restype = None
+        lib.oracle_niche_in.argtypes = [C.c_uint32] + [C.c_float] * 5 + [C.c_uint32, C.c_void_p]
+        lib.oracle_niche_in.restype = None
         lib.oracle_cos_0_pi_n.argtypes = [C.c_void_p, C.c_void_p, C.c_long]
         lib.oracle_atan2_n.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_long]
         lib.oracle_f2h_n.argtypes = [C.c_void_p, C.c_void_p, C.c_long]
@@ -85,3 +89,23 @@ def run_f32(batch, params, run_, threads: int = 1, frames: bool = False):
 
 def run_f64(batch, params, run_, threads: int = 1, frames: bool = False):
     return run(batch, params, run_, "f64", threads, frames)
+
+
+def describe(batch, results, steps: int, rate: int, layout: int = 1):
+    """Archive metrics and niche cells of every creature of `batch` from its
+    results (rows of METRICS_DTYPE), as the reference's host computes them."""
+    from evolution_simulator_gpu_amd import _native as N
+
+    out = np.zeros(batch.count, dtype=N.METRICS_DTYPE)
+    res = np.ascontiguousarray(results)
+    load().oracle_describe(batch.records.ctypes.data, batch.heads.ctypes.data, res.ctypes.data, 0, batch.count,
+                           steps, rate, layout, out.ctypes.data)
+    return out
+
+
+def niche_in(nodes: int, ground_contact: float, gait_frequency: float, aspect_ratio: float, mean_height: float,
+             feet: float, layout: int) -> tuple:
+    """Descriptor::niche_in of one descriptor."""
+    cell = (C.c_uint8 * 6)()
+    load().oracle_niche_in(nodes, ground_contact, gait_frequency, aspect_ratio, mean_height, feet, layout, cell)
+    return tuple(cell)

@@ -882,7 +882,7 @@ static long FN(run_creature)(const FN(World)* w, uint32_t cidx, const uint32_t* 
 
 /* Exported for the oracle's own tests. */
 void FN(ground_export)(uint32_t flags, const evo_params* p, R x, R amp, R qphase, R* h, R* s) {
-    evo_run run = {flags, 60u, 16u, 0u, 0u, 0u};
+    evo_run run = {flags, 60u, 16u, 0u, 0u, 0u, 0u};
     FN(World) w;
     FN(make_world)(&w, p, &run);
     FN(ground)(&w, x, amp, qphase, h, s);

@@ -1,0 +1,140 @@
+"""Archive metrics and niche cells (src/scheduler.rs:859-923,
+src/qd.rs:762-847): the oracle pinned by the reference's own known answers
+and by the logarithm it replaces, then the HIP kernel bit for bit against it.
+"""
+import numpy as np
+import pytest
+
+import evolution_simulator_gpu_amd as E
+import oracle
+from evolution_simulator_gpu_amd import _native as N
+
+ISLAND, GLOBAL = N.LAYOUT_ISLAND, N.LAYOUT_GLOBAL
+
+
+def test_niches_of_the_reference_rebinning_test():
+    """src/qd.rs:2947-3003: contact 0.5, cadence 1.0, height 0.5 m, 2 feet,
+    aspect 1.5 land in [3, 1, 1, 3, 1, 0] of the island layout for 6 and 7
+    nodes; cadence 2.0 moves to cadence bin 2."""
+    assert oracle.niche_in(6, 0.5, 1.0, 1.5, 0.5, 2.0, ISLAND) == (3, 1, 1, 3, 1, 0)
+    assert oracle.niche_in(7, 0.5, 1.0, 1.5, 0.5, 2.0, ISLAND) == (3, 1, 1, 3, 1, 0)
+    assert oracle.niche_in(6, 0.5, 2.0, 1.5, 0.5, 2.0, ISLAND) == (3, 2, 1, 3, 1, 0)
+
+
+def test_every_island_body_class_has_its_own_cell():
+    """tests/search_state.rs:260-320: nine bodies of three sizes and three
+    shapes take the nine body classes; a 6-node body of aspect 0.5 shares
+    the cell of the 5-node body of aspect 0.6."""
+    bodies = [(5, 0.6), (5, 1.8), (5, 4.0), (10, 0.6), (10, 1.8), (10, 4.0), (20, 0.6), (20, 1.8), (20, 4.0)]
+    cells = {oracle.niche_in(n, 0.5, 1.0, a, 0.5, 2.0, ISLAND) for n, a in bodies}
+    assert len(cells) == 9
+    assert {c[2] for c in cells} == {0, 1, 2} and {c[5] for c in cells} == {0, 1, 2}
+    assert oracle.niche_in(6, 0.5, 1.0, 0.5, 0.5, 2.0, ISLAND) == oracle.niche_in(5, 0.5, 1.0, 0.6, 0.5, 2.0, ISLAND)
+
+
+def test_class_edges_of_both_layouts():
+    """src/qd.rs:56-84: a body enters a class at its edge (>=)."""
+    shapes = [(0.89, 0, 0), (0.9, 0, 1), (1.2, 1, 1), (1.4, 1, 2), (2.0, 2, 2), (2.5, 2, 3), (16.0, 2, 3)]
+    for aspect, island, glob in shapes:
+        assert oracle.niche_in(6, 0.5, 1.0, aspect, 0.5, 2.0, ISLAND)[2] == island, aspect
+        assert oracle.niche_in(6, 0.5, 1.0, aspect, 0.5, 2.0, GLOBAL)[2] == glob, aspect
+    sizes = [(8, 0, 0), (9, 1, 1), (10, 1, 1), (11, 1, 2), (12, 2, 2), (13, 2, 2), (14, 2, 3), (32, 2, 3)]
+    for nodes, island, glob in sizes:
+        assert oracle.niche_in(nodes, 0.5, 1.0, 1.0, 0.5, 2.0, ISLAND)[5] == island, nodes
+        assert oracle.niche_in(nodes, 0.5, 1.0, 1.0, 0.5, 2.0, GLOBAL)[5] == glob, nodes
+
+
+def test_height_bins_equal_the_logarithmic_axis():
+    """height_axis (src/qd.rs:837-842) evaluated with float64 logarithms gives
+    the same bin as the oracle's edge comparison, away from the edges."""
+    rng = np.random.default_rng(7)
+    heights = np.concatenate([rng.uniform(0.0, 2.0, 4000), np.exp(rng.uniform(np.log(0.05), np.log(3.0), 4000))])
+    low, high = 0.15, max(0.6 * 2.0, 0.3)
+    axis = np.clip(np.log(np.maximum(heights, low) / low) / np.log(high / low), 0.0, 1.0)
+    want = np.minimum(np.floor(axis * 6), 5).astype(int)
+    edges = low * 2.0 ** (np.arange(1, 6) / 2)
+    clear = np.min(np.abs(heights[:, None] / edges[None, :] - 1.0), axis=1) > 1e-6
+    got = np.array([oracle.niche_in(6, 0.5, 1.0, 1.0, float(h), 2.0, ISLAND)[3] for h in heights])
+    assert clear.sum() > 7900
+    assert np.array_equal(got[clear], want[clear])
+
+
+def test_bins_clamp_and_saturate():
+    """bin (src/qd.rs:843-847): values outside the range fall in the first or
+    last bin, the top of the range in the last, NaN (kept by clamp) in bin 0;
+    feet round and clamp to 1..5 (src/qd.rs:799)."""
+    f = oracle.niche_in
+    assert f(6, -0.3, -1.0, 1.0, 0.0, 0.0, ISLAND)[:2] == (0, 0)
+    assert f(6, 1.0, 6.0, 1.0, 0.5, 2.0, ISLAND)[:2] == (5, 7)
+    assert f(6, 7.0, 19.0, 1.0, 0.5, 2.0, ISLAND)[:2] == (5, 7)
+    assert f(6, float("nan"), 1.0, 1.0, 0.5, 2.0, ISLAND)[0] == 0
+    assert f(6, 0.5, 0.75, 1.0, 0.5, 2.0, ISLAND)[1] == 1  # 0.75 / 6 * 8 = 1
+    assert [f(6, 0.5, 1.0, 1.0, 0.5, feet, ISLAND)[4] for feet in (0, 1, 2, 3, 4, 5, 9)] == [0, 0, 1, 2, 3, 4, 4]
+
+
+def _one_creature_batch(nodes, xs, ys):
+    records = np.zeros(nodes * 6 + (nodes - 1) * 6, np.uint32)
+    for i in range(nodes):
+        records[i * 6 + 3] = np.float32(xs[i]).view(np.uint32)
+        records[i * 6 + 4] = np.float32(ys[i]).view(np.uint32)
+    heads = np.zeros((1, 8), np.uint32)
+    heads[0, 0] = nodes
+    return E.Batch(records, np.zeros(4, np.float32), heads, np.zeros(1, np.uint32))
+
+
+def _result(**fields):
+    r = np.zeros(1, N.RESULT_DTYPE)
+    for k, v in fields.items():
+        r[k] = v
+    return r
+
+
+def test_trial_steps_and_metrics():
+    """trial_steps and to_metrics (src/scheduler.rs:859-923) on hand-made
+    results of a 4-node body over 600 steps at 60 Hz."""
+    b = _one_creature_batch(4, [0.0, 1.0, 2.0, 0.5], [0.1, 0.1, 1.1, 0.6])
+    lift = np.array([0b1011], np.uint32).view(np.float32)[0]
+    code = np.array([16 | (75 << 16)], np.uint32).view(np.float32)[0]
+    m = oracle.describe(b, _result(fitness=2.5, fall_time=1.25, ground_contact=150.0, height_sum=30.0,
+                                   lift_lo=lift, gait_frequency=-1.0, vertical_oscillation=np.nan,
+                                   ground_hi=code), 600, 60)[0]
+    assert m["steps"] == 75 and m["fell"] == 1 and m["screened"] == 0
+    assert m["ground_contact"] == np.float32(0.5) and m["mean_height"] == np.float32(0.4)
+    assert m["feet"] == 3.0 and m["gait_frequency"] == 0.0 and m["vertical_oscillation"] == 0.0
+    assert m["aspect_ratio"] == np.float32(2.0)
+    # a screen stop counts to the screen; a fall beats the screen
+    assert oracle.describe(b, _result(screened=5.0), 600, 60)[0]["steps"] == 300
+    assert oracle.describe(b, _result(screened=5.0, fall_time=2.0), 600, 60)[0]["steps"] == 120
+    # an early fall still counts one step; a late one no more than the trial
+    assert oracle.describe(b, _result(fall_time=0.001), 600, 60)[0]["steps"] == 1
+    assert oracle.describe(b, _result(fall_time=99.0), 600, 60)[0]["steps"] == 600
+    assert oracle.describe(b, _result(), 600, 60)[0]["steps"] == 600
+    # half a step rounds away from zero (Rust's f32::round)
+    assert oracle.describe(b, _result(fall_time=np.float32(2.5) / 60), 600, 60)[0]["steps"] in (2, 3)
+    assert oracle.describe(b, _result(fall_time=0.125), 600, 120)[0]["steps"] == 15
+    # contact shares above 1 clamp; flat and tall bodies clamp their aspect
+    assert oracle.describe(b, _result(ground_contact=1e6), 600, 60)[0]["ground_contact"] == 1.0
+    flat = _one_creature_batch(3, [0.0, 1.0, 2.0], [0.1, 0.1, 0.1])
+    assert oracle.describe(flat, _result(), 600, 60)[0]["aspect_ratio"] == 16.0
+    tall = _one_creature_batch(3, [0.0, 0.0, 0.0], [0.1, 1.0, 2.0])
+    assert oracle.describe(tall, _result(), 600, 60)[0]["aspect_ratio"] == np.float32(0.0625)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("layout", [ISLAND, GLOBAL])
+def test_gpu_metrics_are_bit_exact(gpu, layout):
+    import torch
+
+    for preset, kw in [(0, dict(duration=5.0)), (1, dict(duration=3.0, screen_seconds=1.0, screen_bar=0.05)),
+                       (2, dict(duration=2.0, terrain=0.25, gravity=19.6))]:
+        batch = E.synth(E.synth_spec(preset, 1500))
+        w = E.world(substeps=8, **kw)
+        p, run = E.run_for(batch, w)
+        db = E.DeviceBatch(batch)
+        db.launch(p, run)
+        got = db.describe(p.steps, w.rate, layout)
+        torch.cuda.synchronize()
+        got = got.cpu().numpy().view(N.METRICS_DTYPE)
+        want = oracle.describe(batch, db.results_numpy(), p.steps, w.rate, layout)
+        assert np.array_equal(got.view(np.uint8), want.view(np.uint8)), f"preset {preset}"
+        assert len({tuple(c) for c in got["niche"]}) > 10

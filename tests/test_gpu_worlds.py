@@ -82,8 +82,11 @@ def test_rungs_and_screen_are_bit_exact(gpu):
 def test_recording_frames_and_score(gpu):
     """RECORD: every frame equals the oracle's, and the recorded result is the
     trial's own result bit for bit (tests/gpu_repeatability.rs:76-140)."""
-    batch = E.synth(E.synth_spec(1, 24))
-    for w in (E.world(duration=1.5, substeps=8), E.world(duration=1.0, substeps=16, rate=120, **EFFECTS["hard"])):
+    cases = [(1, E.world(duration=1.5, substeps=8)),  # unrolled recording class
+             (1, E.world(duration=1.0, substeps=16, rate=120, **EFFECTS["hard"])),
+             (2, E.world(duration=1.0, substeps=8, **EFFECTS["hard"]))]  # looped recording class
+    for preset, w in cases:
+        batch = E.synth(E.synth_spec(preset, 24))
         p, run = E.run_for(batch, w, record=True)
         want, wframes, _ = oracle.run_f32(batch, p, run, threads=4, frames=True)
         got, frames = E.simulate(batch, w, record=True)
@@ -118,3 +121,31 @@ def test_a_creature_scores_the_same_in_any_batch(gpu):
     shuffled = big.subset(picks[::-1])
     again = E.simulate(shuffled, w)
     assert np.array_equal(bits(again), bits(together)[picks[::-1]])
+
+
+def test_top_k_replay_capture(gpu):
+    """The k fittest creatures recorded in one launch through gathered head
+    words: the replayed result equals the trial's result bit for bit
+    (tests/gpu_repeatability.rs:76-140) and every frame equals the oracle's
+    recording of that creature."""
+    import torch
+
+    batch = E.synth(E.synth_spec(1, 3000))
+    w = E.world(duration=1.5, substeps=8)
+    p, run = E.run_for(batch, w)
+    db = E.DeviceBatch(batch)
+    db.launch(p, run)
+    pick, frames, res = db.capture_top(16, p, run)
+    torch.cuda.synchronize()
+    pick = pick.cpu().numpy()
+    full = db.results_numpy()
+    assert np.all(full["fitness"][pick] >= np.sort(full["fitness"])[-16])
+    got = res.cpu().numpy().view(N.RESULT_DTYPE)
+    assert np.array_equal(bits(got), bits(full)[pick])
+    sub = batch.subset(pick)
+    q, r = E.run_for(sub, w, record=True)
+    q.stride = p.stride
+    _, wframes, _ = oracle.run_f32(sub, q, r, threads=4, frames=True)
+    f = frames.cpu().numpy()
+    nan = np.isnan(f) & np.isnan(wframes)
+    assert np.array_equal(f.view(np.uint32)[~nan], wframes.view(np.uint32)[~nan])
