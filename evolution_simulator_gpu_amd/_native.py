@@ -92,6 +92,18 @@ class SynthSpec(C.Structure):
     ]
 
 
+class ClassRange(C.Structure):
+    _fields_ = [("base", C.c_uint32), ("count", C.c_uint32), ("max_nodes", C.c_uint32), ("max_muscles", C.c_uint32),
+                ("no_tendons", C.c_uint32), ("grid", C.c_int32)]
+
+
+class WildMix(C.Structure):
+    _fields_ = [("count", C.c_uint32), ("effect", C.c_uint32 * 3), ("level", C.c_uint32 * 3)]
+
+
+EFFECTS = 16
+WILD_ISLANDS = 100
+
 RESULT_DTYPE = np.dtype([(name, np.float32) for name in RESULT_FIELDS])
 assert RESULT_DTYPE.itemsize == 76
 
@@ -123,6 +135,19 @@ SIGNATURES = {
     "evo_synth_fill": ([C.POINTER(SynthSpec), _P, _P, _P, _P, _P, _P], C.c_int),
     "evo_advance_device": ([_P, _P, _P, _P, _P, C.POINTER(Params), C.POINTER(Run), _P, _P], C.c_int),
     "evo_describe_device": ([_P, _P, _P, _u32, _u32, _u32, _u32, _u32, _P, _P], C.c_int),
+    "evo_advance_worlds_device": ([_P, _P, _P, _P, _P, C.POINTER(Params), C.POINTER(Run), _P, _P, _P, _u32, C.c_int32,
+                                   _P], C.c_int),
+    "evo_kernel_capacity": ([C.POINTER(Run), C.c_int32], C.c_int),
+    "evo_advance_ranges_device": ([_P, _P, _P, _P, _P, C.POINTER(Params), C.POINTER(Run), _P, _u32, _P, _P, _P, _u32,
+                                   _P], C.c_int),
+    "evo_graph_begin": ([_P], C.c_int),
+    "evo_graph_end": ([_P, C.POINTER(C.c_void_p)], C.c_int),
+    "evo_graph_launch": ([_P, _P], C.c_int),
+    "evo_graph_destroy": ([_P], None),
+    "evo_wild_levels": ([_u32, _P], C.c_int),
+    "evo_wild_world": ([C.POINTER(WorldConfig), C.POINTER(WildMix), C.POINTER(WorldConfig)], C.c_int),
+    "evo_island_worlds": ([_u64, _u32, C.POINTER(WorldConfig), _P], C.c_int),
+    "evo_splitmix_finalize": ([_u64], _u64),
     "evo_validate_batch": ([_P, _u64, _P, _u64, _P, _u32, C.POINTER(_u32)], C.c_int),
     "evo_max_muscles": ([_P, _u32, _u32], _u32),
     "evo_build_info": ([], C.c_char_p),

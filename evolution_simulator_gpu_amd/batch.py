@@ -86,6 +86,70 @@ def params_for(w: N.WorldConfig, base: int, count: int, stride: int = 0) -> tupl
     return p, run
 
 
+def wild_levels(n: int = N.WILD_ISLANDS) -> list[list[tuple[int, int]]]:
+    """The wild islands' effect mixes (src/environment.rs:479-521): per
+    island a list of (effect index, level)."""
+    mixes = (N.WildMix * n)()
+    if N.lib().evo_wild_levels(n, mixes) != N.EVO_OK:
+        raise ValueError("invalid island count")
+    return [[(m.effect[i], m.level[i]) for i in range(m.count)] for m in mixes]
+
+
+def wild_world(base: N.WorldConfig, mix: list[tuple[int, int]]) -> N.WorldConfig:
+    """`base` with every effect calm but the island's mix
+    (src/environment.rs:529-542)."""
+    m = N.WildMix()
+    m.count = len(mix)
+    for i, (e, level) in enumerate(mix):
+        m.effect[i], m.level[i] = e, level
+    out = N.WorldConfig()
+    if N.lib().evo_wild_world(C.byref(base), C.byref(m), C.byref(out)) != N.EVO_OK:
+        raise ValueError("invalid effect mix")
+    return out
+
+
+def island_worlds(n: int, seed: int = 0, base: N.WorldConfig | None = None) -> list[N.WorldConfig]:
+    """n island worlds of the batched benchmark drawn from `seed`: ground
+    bumps, gravity, grip, water and headwind per island."""
+    base = base if base is not None else world()
+    out = (N.WorldConfig * n)()
+    if N.lib().evo_island_worlds(seed, n, C.byref(base), out) != N.EVO_OK:
+        raise ValueError("invalid island worlds")
+    return list(out)
+
+
+class WorldTable:
+    """The worlds of a batch whose creatures run in different worlds: one
+    launch-parameter record and effect-bit word per world, and the world of
+    each packed creature. All worlds share the step rate, substeps and trial
+    length of the first."""
+
+    def __init__(self, worlds: list[N.WorldConfig], world_of_packed: np.ndarray, count: int, stride: int):
+        if not worlds or len(worlds) > 65535:
+            raise ValueError("between 1 and 65535 worlds")
+        first = worlds[0]
+        self.params = (N.Params * len(worlds))()
+        self.flags = np.zeros(len(worlds), np.uint32)
+        flag_union = 0
+        for k, w in enumerate(worlds):
+            if (w.rate, w.substeps, w.duration) != (first.rate, first.substeps, first.duration):
+                raise ValueError("every world of a table needs the same rate, substeps and duration")
+            p, run = params_for(w, 0, count, stride)
+            self.params[k] = p
+            self.flags[k] = run.flags
+            flag_union |= run.flags
+        self.world_of = np.ascontiguousarray(world_of_packed, dtype=np.uint16)
+        if self.world_of.shape != (count,) or int(self.world_of.max(initial=0)) >= len(worlds):
+            raise ValueError("one world index per creature, each below the number of worlds")
+        self.worlds = list(worlds)
+        self.flag_union = flag_union
+
+    def run_for(self, batch: "Batch") -> tuple[N.Params, N.Run]:
+        p, run = run_for(batch, self.worlds[0])
+        run.flags = self.flag_union
+        return p, run
+
+
 def synth_spec(preset: int, count: int | None = None, **overrides) -> N.SynthSpec:
     s = N.SynthSpec()
     if N.lib().evo_synth_preset(preset, C.byref(s)) != N.EVO_OK:
@@ -163,7 +227,8 @@ class DeviceBatch:
         self.muscles = torch.from_numpy(batch.muscles).to(device)
         self.heads = torch.from_numpy(batch.heads.reshape(-1).view(np.int32)).to(device)
         self.results = torch.zeros(batch.count * 19, dtype=torch.float32, device=device)
-        self.counter = torch.zeros(4, dtype=torch.int32, device=device)
+        self.counter = torch.zeros(8, dtype=torch.int32, device=device)
+        self.world_params = None
         self.frames = None
 
     def alloc_frames(self, p: N.Params, rate: int) -> None:
@@ -194,6 +259,56 @@ class DeviceBatch:
                 self.frames.data_ptr() if self.frames is not None else None, C.c_void_p(stream.cuda_stream))
             if rc != N.EVO_OK:
                 raise RuntimeError(f"evo_advance_device failed: {rc}")
+
+    def set_worlds(self, table: WorldTable) -> None:
+        """Uploads a world table: every later launch_worlds / launch_ranges
+        runs each creature in its own world."""
+        import torch
+
+        raw = np.frombuffer(bytes(table.params), dtype=np.uint8)
+        self.world_params = torch.from_numpy(raw.copy()).to(self.device)
+        self.world_flags = torch.from_numpy(table.flags.view(np.int32).copy()).to(self.device)
+        self.world_of = torch.from_numpy(table.world_of.view(np.int16).copy()).to(self.device)
+        self.n_worlds = len(table.worlds)
+
+    def _world_ptrs(self):
+        if getattr(self, "world_params", None) is None:
+            return None, None, None, 0
+        return (self.world_params.data_ptr(), self.world_flags.data_ptr(), self.world_of.data_ptr(), self.n_worlds)
+
+    def launch_worlds(self, p: N.Params, run: N.Run, grid: int = 0, stream=None) -> None:
+        """One launch over creatures [p.base, p.base + p.count), each in its
+        own world from the uploaded table."""
+        import torch
+
+        if stream is None:
+            stream = torch.cuda.current_stream(self.records.device)
+        wp, wf, wo, nw = self._world_ptrs()
+        if wp is None:
+            raise RuntimeError("no world table uploaded")
+        rc = N.lib().evo_advance_worlds_device(
+            self.records.data_ptr(), self.muscles.data_ptr(), self.heads.data_ptr(), self.results.data_ptr(),
+            self.counter.data_ptr(), C.byref(p), C.byref(run), wp, wf, wo, nw, grid, C.c_void_p(stream.cuda_stream))
+        if rc != N.EVO_OK:
+            raise RuntimeError(f"evo_advance_worlds_device failed: {rc}")
+
+    def launch_ranges(self, p: N.Params, run: N.Run, ranges: list[tuple[int, int, int, int, int, int]],
+                      worlds: bool = True, stream=None) -> None:
+        """Concurrent persistent kernels, one per (base, count, max_nodes,
+        max_muscles, no_tendons, grid) range, forked from and joined back
+        into `stream`; with `worlds` each creature runs in its table world."""
+        import torch
+
+        if stream is None:
+            stream = torch.cuda.current_stream(self.records.device)
+        arr = (N.ClassRange * len(ranges))(*[N.ClassRange(*r) for r in ranges])
+        wp, wf, wo, nw = self._world_ptrs() if worlds else (None, None, None, 0)
+        rc = N.lib().evo_advance_ranges_device(
+            self.records.data_ptr(), self.muscles.data_ptr(), self.heads.data_ptr(), self.results.data_ptr(),
+            self.counter.data_ptr(), C.byref(p), C.byref(run), arr, len(ranges), wp, wf, wo, nw,
+            C.c_void_p(stream.cuda_stream))
+        if rc != N.EVO_OK:
+            raise RuntimeError(f"evo_advance_ranges_device failed: {rc}")
 
     def describe(self, steps: int, rate: int, layout: int = N.LAYOUT_GLOBAL, stream=None, out=None):
         """Archive metrics and niche cells of every creature from the last
@@ -237,7 +352,7 @@ class DeviceBatch:
         frames = torch.zeros(k * fpc * p.stride * 2, dtype=torch.float32, device=self.device)
         results = torch.empty(k * len(N.RESULT_FIELDS), dtype=torch.float32, device=self.device)
         rc = N.lib().evo_advance_device(self.records.data_ptr(), self.muscles.data_ptr(), heads.data_ptr(),
-                                        results.data_ptr(), self.counter.data_ptr() + 12, C.byref(q), C.byref(r),
+                                        results.data_ptr(), self.counter.data_ptr() + 28, C.byref(q), C.byref(r),
                                         frames.data_ptr(), C.c_void_p(stream.cuda_stream))
         if rc != N.EVO_OK:
             raise RuntimeError(f"evo_advance_device (recording) failed: {rc}")
@@ -325,3 +440,56 @@ def from_bodies(bodies: list[dict], ids: list[int] | None = None, flags: list[in
     muscles_all = np.concatenate(muss) if mf else np.zeros(4, np.float32)
     return Batch(np.concatenate(recs), muscles_all.astype(np.float32), np.stack(heads),
                  np.arange(len(bodies), dtype=np.uint32))
+
+
+def class_ranges(batch: Batch, grids: tuple[int, int] = (0, 0)) -> list[tuple[int, int, int, int, int, int]]:
+    """The two body-size classes of a class-major packed batch (synthetic
+    spec sorted = 2): bodies of at most 12 nodes and 24 muscles, then the
+    rest, as (base, count, max_nodes, max_muscles, no_tendons, grid)."""
+    n = batch.nodes().astype(np.int64)
+    m = batch.muscle_counts().astype(np.int64)
+    big = (n > 12) | (m > 24)
+    cut = int(np.argmax(big)) if big.any() else batch.count
+    if big[:cut].any() or not big[cut:].all():
+        raise ValueError("the batch is not packed class-major")
+    out = []
+    for k, (a, b) in enumerate(((0, cut), (cut, batch.count))):
+        if b > a:
+            # a range's muscle records are contiguous in packed order
+            lo = int(batch.heads[a, 4])
+            hi = int(batch.heads[b - 1, 4]) + int(m[b - 1]) * N.MUSCLE_WORDS
+            rows = batch.muscles[lo:hi].reshape(-1, N.MUSCLE_WORDS)
+            free = bool(np.all(rows[:, 6] == 0.0))
+            out.append((a, b - a, int(n[a:b].max()), int(m[a:b].max()), int(free), int(grids[k])))
+    return out
+
+
+class Graph:
+    """A HIP graph of this library's launches (evo_graph_*): `capture` runs
+    `work(stream)` under capture on a side stream, `replay` launches the
+    graph."""
+
+    def __init__(self):
+        self.handle = C.c_void_p(None)
+
+    def capture(self, work, stream) -> None:
+        lib = N.lib()
+        rc = lib.evo_graph_begin(C.c_void_p(stream.cuda_stream))
+        if rc != N.EVO_OK:
+            raise RuntimeError(f"evo_graph_begin failed: {rc}")
+        try:
+            work(stream)
+        finally:
+            rc = lib.evo_graph_end(C.c_void_p(stream.cuda_stream), C.byref(self.handle))
+        if rc != N.EVO_OK:
+            raise RuntimeError(f"evo_graph_end failed: {rc}")
+
+    def replay(self, stream) -> None:
+        rc = N.lib().evo_graph_launch(self.handle, C.c_void_p(stream.cuda_stream))
+        if rc != N.EVO_OK:
+            raise RuntimeError(f"evo_graph_launch failed: {rc}")
+
+    def __del__(self):
+        if self.handle and self.handle.value:
+            N.lib().evo_graph_destroy(self.handle)
+            self.handle = C.c_void_p(None)

@@ -11,17 +11,19 @@
 #include "../../include/evo_engine.h"
 #include "launch.hpp"
 #include <cstdlib>
+#include <mutex>
 
 namespace {
 
-using evo::dev::Launcher;
+using evo::dev::Instance;
+using evo::dev::WorldTable;
 
-Launcher pick(const evo_run& run) {
-    if (run.rate == 60u && run.substeps == 8u) { return evo::dev::pick_r60_s8(run); }
-    if (run.rate == 60u && run.substeps == 16u) { return evo::dev::pick_r60_s16(run); }
-    if (run.rate == 120u && run.substeps == 8u) { return evo::dev::pick_r120_s8(run); }
-    if (run.rate == 120u && run.substeps == 16u) { return evo::dev::pick_r120_s16(run); }
-    return nullptr;
+Instance pick(const evo_run& run, bool table) {
+    if (run.rate == 60u && run.substeps == 8u) { return evo::dev::pick_r60_s8(run, table); }
+    if (run.rate == 60u && run.substeps == 16u) { return evo::dev::pick_r60_s16(run, table); }
+    if (run.rate == 120u && run.substeps == 8u) { return evo::dev::pick_r120_s8(run, table); }
+    if (run.rate == 120u && run.substeps == 16u) { return evo::dev::pick_r120_s16(run, table); }
+    return Instance{nullptr, nullptr};
 }
 
 int grid_cap_from_env() {
@@ -33,6 +35,58 @@ int grid_cap_from_env() {
     return cap;
 }
 
+int to_code(hipError_t e) {
+    if (e == hipErrorOutOfMemory) { return EVO_ERR_OOM; }
+    return e == hipSuccess ? EVO_OK : EVO_ERR_DEVICE;
+}
+
+int check_args(const uint32_t* d_records, const float* d_muscles, const uint32_t* d_heads, const evo_result* d_results,
+               const uint32_t* d_counter, const evo_params* p, const evo_run* run) {
+    if (!p || !run || !d_records || !d_muscles || !d_heads || !d_results || !d_counter) { return EVO_ERR_INVALID; }
+    if (p->steps == 0u || (run->flags & ~EVO_ALL_FLAGS) != 0u) { return EVO_ERR_INVALID; }
+    if (run->max_nodes > EVO_MAX_NODES || run->max_muscles > EVO_MAX_MUSCLES) { return EVO_ERR_INVALID; }
+    return EVO_OK;
+}
+
+int launch(const Instance& in, const uint32_t* d_records, const float* d_muscles, const uint32_t* d_heads,
+           evo_result* d_results, uint32_t* d_counter, const evo_params& p, const evo_run& run, const WorldTable& wt,
+           float* d_frames, int grid, hipStream_t stream) {
+    if (!in.launch) { return EVO_ERR_INVALID; }
+    const int cap = grid_cap_from_env();
+    if (cap > 0) { grid = grid > 0 ? std::min(grid, cap) : cap; }
+    const uint64_t fpc = static_cast<uint64_t>(evo_settle_frames(run.rate)) + p.steps + 1u;
+    return to_code(in.launch(d_records, d_muscles, reinterpret_cast<const uint4*>(d_heads), d_results, d_counter, p,
+                             run.flags, wt, reinterpret_cast<float2*>(d_frames), fpc, stream, grid));
+}
+
+// Side streams for the concurrent ranges of one generation, created once per
+// device at the default priority.
+constexpr int kSideStreams = 4;
+struct Side {
+    hipStream_t stream[kSideStreams] = {};
+    hipEvent_t fork = nullptr;
+    hipEvent_t join[kSideStreams] = {};
+    bool ready = false;
+};
+std::mutex side_mutex;
+Side side_of[64];
+
+Side* side_streams() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) { return nullptr; }
+    std::lock_guard<std::mutex> lock(side_mutex);
+    Side& s = side_of[dev];
+    if (!s.ready) {
+        if (hipEventCreateWithFlags(&s.fork, hipEventDisableTiming) != hipSuccess) { return nullptr; }
+        for (int i = 0; i < kSideStreams; i++) {
+            if (hipStreamCreateWithFlags(&s.stream[i], hipStreamNonBlocking) != hipSuccess) { return nullptr; }
+            if (hipEventCreateWithFlags(&s.join[i], hipEventDisableTiming) != hipSuccess) { return nullptr; }
+        }
+        s.ready = true;
+    }
+    return &s;
+}
+
 }  // namespace
 
 extern "C" {
@@ -40,24 +94,133 @@ extern "C" {
 int evo_advance_device(const uint32_t* d_records, const float* d_muscles, const uint32_t* d_heads,
                        evo_result* d_results, uint32_t* d_counter, const evo_params* p, const evo_run* run,
                        float* d_frames, void* stream) {
-    if (!p || !run || !d_records || !d_muscles || !d_heads || !d_results || !d_counter) { return EVO_ERR_INVALID; }
+    const int rc = check_args(d_records, d_muscles, d_heads, d_results, d_counter, p, run);
+    if (rc != EVO_OK) { return rc; }
     if (run->record && !d_frames) { return EVO_ERR_INVALID; }
-    if (p->steps == 0u || (run->flags & ~EVO_ALL_FLAGS) != 0u) { return EVO_ERR_INVALID; }
-    if (run->max_nodes > EVO_MAX_NODES || run->max_muscles > EVO_MAX_MUSCLES) { return EVO_ERR_INVALID; }
     if (p->count == 0u) { return EVO_OK; }
-    const Launcher l = pick(*run);
-    if (!l) { return EVO_ERR_INVALID; }
-    const uint64_t fpc = static_cast<uint64_t>(evo_settle_frames(run->rate)) + p->steps + 1u;
-    const hipError_t e = l(d_records, d_muscles, reinterpret_cast<const uint4*>(d_heads), d_results, d_counter, *p,
-                           run->flags, reinterpret_cast<float2*>(d_frames), fpc,
-                           reinterpret_cast<hipStream_t>(stream), grid_cap_from_env());
-    if (e == hipErrorOutOfMemory) { return EVO_ERR_OOM; }
-    return e == hipSuccess ? EVO_OK : EVO_ERR_DEVICE;
+    const WorldTable none{nullptr, nullptr, nullptr};
+    return launch(pick(*run, false), d_records, d_muscles, d_heads, d_results, d_counter, *p, *run, none, d_frames, 0,
+                  reinterpret_cast<hipStream_t>(stream));
+}
+
+int evo_advance_worlds_device(const uint32_t* d_records, const float* d_muscles, const uint32_t* d_heads,
+                              evo_result* d_results, uint32_t* d_counter, const evo_params* p, const evo_run* run,
+                              const evo_params* d_worlds, const uint32_t* d_world_flags,
+                              const uint16_t* d_world_of, uint32_t n_worlds, int32_t grid, void* stream) {
+    const int rc = check_args(d_records, d_muscles, d_heads, d_results, d_counter, p, run);
+    if (rc != EVO_OK) { return rc; }
+    if (run->record || !d_worlds || !d_world_flags || !d_world_of || n_worlds == 0u || grid < 0) {
+        return EVO_ERR_INVALID;
+    }
+    if (p->count == 0u) { return EVO_OK; }
+    const WorldTable wt{d_worlds, d_world_flags, d_world_of};
+    return launch(pick(*run, true), d_records, d_muscles, d_heads, d_results, d_counter, *p, *run, wt, nullptr, grid,
+                  reinterpret_cast<hipStream_t>(stream));
+}
+
+int evo_kernel_capacity(const evo_run* run, int32_t table) {
+    if (!run) { return EVO_ERR_INVALID; }
+    const Instance in = pick(*run, table != 0);
+    if (!in.capacity) { return EVO_ERR_INVALID; }
+    return in.capacity();
+}
+
+int evo_advance_ranges_device(const uint32_t* d_records, const float* d_muscles, const uint32_t* d_heads,
+                              evo_result* d_results, uint32_t* d_counters, const evo_params* p, const evo_run* run,
+                              const evo_class_range* ranges, uint32_t n_ranges, const evo_params* d_worlds,
+                              const uint32_t* d_world_flags, const uint16_t* d_world_of, uint32_t n_worlds,
+                              void* stream) {
+    const int rc = check_args(d_records, d_muscles, d_heads, d_results, d_counters, p, run);
+    if (rc != EVO_OK) { return rc; }
+    if (!ranges || n_ranges == 0u || n_ranges > static_cast<uint32_t>(kSideStreams) || run->record) {
+        return EVO_ERR_INVALID;
+    }
+    const bool table = d_worlds != nullptr;
+    if (table && (!d_world_flags || !d_world_of || n_worlds == 0u)) { return EVO_ERR_INVALID; }
+    for (uint32_t i = 0; i < n_ranges; i++) {
+        const evo_class_range& r = ranges[i];
+        if (r.grid < 0 || r.max_nodes > EVO_MAX_NODES || r.max_muscles > EVO_MAX_MUSCLES) { return EVO_ERR_INVALID; }
+        if (static_cast<uint64_t>(r.base) + r.count > static_cast<uint64_t>(p->base) + p->count || r.base < p->base) {
+            return EVO_ERR_INVALID;
+        }
+    }
+    Side* side = side_streams();
+    if (!side) { return EVO_ERR_DEVICE; }
+    const hipStream_t main = reinterpret_cast<hipStream_t>(stream);
+    const WorldTable wt{d_worlds, d_world_flags, d_world_of};
+    hipError_t e = hipEventRecord(side->fork, main);
+    if (e != hipSuccess) { return to_code(e); }
+    int result = EVO_OK;
+    uint32_t forked = 0;
+    for (uint32_t i = 0; i < n_ranges; i++) {
+        const evo_class_range& r = ranges[i];
+        e = hipStreamWaitEvent(side->stream[i], side->fork, 0);
+        if (e != hipSuccess) {
+            result = to_code(e);
+            break;
+        }
+        forked = i + 1u;
+        if (r.count == 0u) { continue; }
+        evo_params q = *p;
+        q.base = r.base;
+        q.count = r.count;
+        evo_run rr = *run;
+        rr.max_nodes = r.max_nodes;
+        rr.max_muscles = r.max_muscles;
+        rr.no_tendons = r.no_tendons;
+        result = launch(pick(rr, table), d_records, d_muscles, d_heads, d_results, d_counters + i, q, rr, wt, nullptr,
+                        r.grid, side->stream[i]);
+        if (result != EVO_OK) { break; }
+    }
+    // join every forked stream back, also after a failure, so the caller's
+    // stream never runs ahead of work it started
+    for (uint32_t i = 0; i < forked; i++) {
+        hipEventRecord(side->join[i], side->stream[i]);
+        hipStreamWaitEvent(main, side->join[i], 0);
+    }
+    return result;
+}
+
+struct evo_graph {
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+};
+
+int evo_graph_begin(void* stream) {
+    return to_code(hipStreamBeginCapture(reinterpret_cast<hipStream_t>(stream), hipStreamCaptureModeThreadLocal));
+}
+
+int evo_graph_end(void* stream, evo_graph** out) {
+    if (!out) { return EVO_ERR_INVALID; }
+    *out = nullptr;
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(reinterpret_cast<hipStream_t>(stream), &g);
+    if (e != hipSuccess) { return to_code(e); }
+    hipGraphExec_t x = nullptr;
+    e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        hipGraphDestroy(g);
+        return to_code(e);
+    }
+    *out = new evo_graph{g, x};
+    return EVO_OK;
+}
+
+int evo_graph_launch(evo_graph* g, void* stream) {
+    if (!g) { return EVO_ERR_INVALID; }
+    return to_code(hipGraphLaunch(g->exec, reinterpret_cast<hipStream_t>(stream)));
+}
+
+void evo_graph_destroy(evo_graph* g) {
+    if (!g) { return; }
+    hipGraphExecDestroy(g->exec);
+    hipGraphDestroy(g->graph);
+    delete g;
 }
 
 const char* evo_build_info(void) {
-    return "evo-mi355x gfx950: rates {60,120} x substeps {8,16} x worlds {flat, any} x sizes "
-           "{12n/8m unrolled, 12n/24m unrolled, 32n/96m looped}; recording: any world, looped";
+    return "evo-mi355x gfx950: rates {60,120} x substeps {8,16} x worlds {flat, any, per-creature table} x sizes "
+           "{12n/8m unrolled, 12n/24m unrolled, 32n/96m looped}; recording: any world, 12n/24m unrolled or looped";
 }
 
 }  // extern "C"

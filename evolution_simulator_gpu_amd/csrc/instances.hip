@@ -1,9 +1,10 @@
 // The kernel instances of one (steps per second, substeps) pair, chosen by
 // EVO_RATE and EVO_SUB at compile time. The calm flat world (ground only, the
 // world of the plain benchmarks) has its own instances; one any-world
-// instance per body-size class reads the effect bits from a scalar register.
-// Recording kernels run one or a few creatures, so they exist only in the
-// run-time-loop class.
+// instance per body-size class reads the effect bits from a scalar register,
+// and one world-table instance per class (mid and large) lets every creature
+// read its own world. Recording kernels run one or a few creatures in one
+// world: the unrolled mid class and the run-time-loop class.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -45,49 +46,57 @@ int max_resident_blocks() {
 
 template <class C>
 hipError_t launch(const uint32_t* records, const float* muscles, const uint4* heads, evo_result* results,
-                  uint32_t* counter, const evo_params& p, uint32_t flags, float2* frames, uint64_t fpc,
-                  hipStream_t stream, int grid_cap) {
+                  uint32_t* counter, const evo_params& p, uint32_t flags, const WorldTable& worlds, float2* frames,
+                  uint64_t fpc, hipStream_t stream, int grid) {
     const int need = static_cast<int>((p.count + C::BLOCK - 1u) / C::BLOCK);
-    int grid = std::min(need, max_resident_blocks<C>());
-    if (grid_cap > 0) { grid = std::min(grid, grid_cap); }
+    grid = std::min(need, grid > 0 ? grid : max_resident_blocks<C>());
     grid = std::max(grid, 1);
     hipLaunchKernelGGL(clear_words<0>, dim3(1), dim3(64), 0, stream, counter, 1u);
     hipLaunchKernelGGL(advance_kernel<C>, dim3(grid), dim3(C::BLOCK), 0, stream, records, muscles, heads, results,
-                       counter, p, flags, frames, fpc);
+                       counter, p, flags, worlds, frames, fpc);
     return hipGetLastError();
 }
 
+template <class C>
+Instance instance() {
+    return Instance{&launch<C>, &max_resident_blocks<C>};
+}
+
 template <uint32_t WORLD, bool TENDON>
-Launcher pick_unrolled(const evo_run& run) {
+Instance pick_unrolled(const evo_run& run) {
     constexpr uint32_t R = EVO_RATE, S = EVO_SUB;
-    if (run.max_nodes <= static_cast<uint32_t>(kSmallN) && run.max_muscles <= static_cast<uint32_t>(kSmallM)) {
-        return &launch<Cfg<R, S, WORLD, kSmallN, kSmallM, false, 128, true, TENDON>>;
+    if constexpr (WORLD != kTableWorld) {
+        if (run.max_nodes <= static_cast<uint32_t>(kSmallN) && run.max_muscles <= static_cast<uint32_t>(kSmallM)) {
+            return instance<Cfg<R, S, WORLD, kSmallN, kSmallM, false, 128, true, TENDON>>();
+        }
     }
-    return &launch<Cfg<R, S, WORLD, kMidN, kMidM, false, 128, true, TENDON>>;
+    return instance<Cfg<R, S, WORLD, kMidN, kMidM, false, 128, true, TENDON>>();
 }
 
 template <uint32_t WORLD>
-Launcher pick_size(const evo_run& run) {
+Instance pick_size(const evo_run& run) {
     constexpr uint32_t R = EVO_RATE, S = EVO_SUB;
     const bool known = run.max_nodes != 0u;
     if (known && run.max_nodes <= static_cast<uint32_t>(kMidN) && run.max_muscles <= static_cast<uint32_t>(kMidM)) {
         return run.no_tendons ? pick_unrolled<WORLD, false>(run) : pick_unrolled<WORLD, true>(run);
     }
-    return &launch<Cfg<R, S, WORLD, EVO_MAX_NODES, EVO_MAX_MUSCLES, false, 64, false, true>>;
+    return instance<Cfg<R, S, WORLD, EVO_MAX_NODES, EVO_MAX_MUSCLES, false, 64, false, true>>();
 }
 
 }  // namespace
 
-Launcher EVO_PICK(const evo_run& run) {
+Instance EVO_PICK(const evo_run& run, bool table) {
     if (run.record) {
+        if (table) { return Instance{nullptr, nullptr}; }
         // replays of bodies that fit the unrolled mid class (the usual top-k
         // capture) run unrolled; larger or unknown bodies run the looped class
         if (run.max_nodes != 0u && run.max_nodes <= static_cast<uint32_t>(kMidN) &&
             run.max_muscles <= static_cast<uint32_t>(kMidM)) {
-            return &launch<Cfg<EVO_RATE, EVO_SUB, kAnyWorld, kMidN, kMidM, true, 128, true, true>>;
+            return instance<Cfg<EVO_RATE, EVO_SUB, kAnyWorld, kMidN, kMidM, true, 128, true, true>>();
         }
-        return &launch<Cfg<EVO_RATE, EVO_SUB, kAnyWorld, EVO_MAX_NODES, EVO_MAX_MUSCLES, true, 64, false, true>>;
+        return instance<Cfg<EVO_RATE, EVO_SUB, kAnyWorld, EVO_MAX_NODES, EVO_MAX_MUSCLES, true, 64, false, true>>();
     }
+    if (table) { return pick_size<kTableWorld>(run); }
     return run.flags == kFlat ? pick_size<kFlat>(run) : pick_size<kAnyWorld>(run);
 }
 

@@ -11,16 +11,35 @@
 namespace evo {
 namespace dev {
 
+// The worlds of a batch whose creatures run in different worlds: world k's
+// launch parameters and effect bits, and the world of each packed creature.
+// All three live in device memory; `params` is null for a one-world batch.
+struct WorldTable {
+    const evo_params* params;
+    const uint32_t* flags;
+    const uint16_t* of;
+};
+
 using Launcher = hipError_t (*)(const uint32_t* records, const float* muscles, const uint4* heads,
                                 evo_result* results, uint32_t* counter, const evo_params& p, uint32_t flags,
-                                float2* frames, uint64_t frames_per_creature, hipStream_t stream, int grid_cap);
+                                const WorldTable& worlds, float2* frames, uint64_t frames_per_creature,
+                                hipStream_t stream, int grid);
 
-// The launcher for a batch with this run's world, recording switch and body
-// sizes, or nullptr when none is compiled in.
-Launcher pick_r60_s8(const evo_run& run);
-Launcher pick_r60_s16(const evo_run& run);
-Launcher pick_r120_s8(const evo_run& run);
-Launcher pick_r120_s16(const evo_run& run);
+// A compiled kernel instance: its launcher (grid = blocks to launch, or 0
+// for as many as fit on the GPU at once) and the number of its blocks that
+// fit on the GPU at once.
+struct Instance {
+    Launcher launch;
+    int (*capacity)();
+};
+
+// The instance for a batch with this run's world, recording switch and body
+// sizes (`table`: every creature reads its own world from a WorldTable), or
+// {nullptr, nullptr} when none is compiled in.
+Instance pick_r60_s8(const evo_run& run, bool table);
+Instance pick_r60_s16(const evo_run& run, bool table);
+Instance pick_r120_s8(const evo_run& run, bool table);
+Instance pick_r120_s16(const evo_run& run, bool table);
 
 // Body-size classes: the largest node and muscle counts each unrolls to.
 // The last class runs its loops at run time and has no such limit below the

@@ -38,12 +38,15 @@
 #include <type_traits>
 
 #include "../../include/evo_types.h"
+#include "launch.hpp"
 #include "devmath.hpp"
 
 namespace evo {
 namespace dev {
 
 constexpr uint32_t kAnyWorld = 0xffffffffu;
+// Every creature reads its own world from a WorldTable.
+constexpr uint32_t kTableWorld = 0xfffffffeu;
 
 template <uint32_t RATE, uint32_t SUB, uint32_t WORLD, int MAXN_, int MAXM_, bool RECORD, int BLOCK_, bool UNROLLED,
           bool TENDON>
@@ -81,7 +84,7 @@ struct Cfg {
 // the launch's flag word (a scalar register) for the any-world kernel.
 template <class C, uint32_t F>
 __device__ __forceinline__ bool has(uint32_t flags) {
-    if constexpr (C::world != kAnyWorld) {
+    if constexpr (C::world != kAnyWorld && C::world != kTableWorld) {
         return (C::world & F) != 0u;
     } else {
         return (flags & F) != 0u;
@@ -250,7 +253,10 @@ struct Cols {
     float* off;
     __device__ __forceinline__ float4& p(uint32_t i) const { return pv[i * C::BLOCK]; }
     __device__ __forceinline__ float2& f(uint32_t i) const { return fp[i * C::BLOCK]; }
-    __device__ __forceinline__ float& o(uint32_t k) const { return off[k * C::BLOCK]; }
+    // muscle rhythm offsets: an LDS column in the unrolled classes, the
+    // lane's own (scratch) array in the run-time-loop class, whose LDS holds
+    // only the node state so that more of its blocks fit on a CU
+    __device__ __forceinline__ float& o(uint32_t k) const { return off[k * (C::unrolled ? C::BLOCK : 1u)]; }
 };
 
 // The wave's largest node, bone and muscle counts: the scalar loop bounds.
@@ -266,7 +272,7 @@ struct Trial {
     uint32_t contact_bits, lift_bits, ground_bits;
     float head_shake, head_at_x, head_at_y, head_vel_x, head_vel_y;
     uint32_t rung_speed, rung_early, rung_late, rung_bits;
-    uint32_t flags, step, mus_off, cidx;
+    uint32_t flags, step, mus_off, cidx, world;
     float rung_x, period_f;
     bool limp, scoring, live;
     evo_result kept;
@@ -1146,17 +1152,23 @@ __global__ void __launch_bounds__(C::BLOCK) advance_kernel(const uint32_t* __res
                                                           const uint4* __restrict__ heads,
                                                           evo_result* __restrict__ results,
                                                           uint32_t* __restrict__ counter, const evo_params p,
-                                                          uint32_t flags, float2* __restrict__ frames,
+                                                          uint32_t flags, const WorldTable worlds,
+                                                          float2* __restrict__ frames,
                                                           uint64_t frames_per_creature) {
     __shared__ float4 s_pv[(C::MAXN + 1) * C::BLOCK];
     __shared__ float2 s_fp[(C::MAXN + 1) * C::BLOCK];
-    __shared__ float s_off[(C::MAXM > 0 ? C::MAXM : 1) * C::BLOCK];
+    __shared__ float s_off[(C::unrolled && C::MAXM > 0 ? C::MAXM : 1) * C::BLOCK];
+    float l_off[C::unrolled ? 1 : C::MAXM];
     const uint32_t lane = threadIdx.x & 63u;
     const uint64_t below = (1ull << lane) - 1ull;
     Cols<C> s;
     s.pv = s_pv + threadIdx.x;
     s.fp = s_fp + threadIdx.x;
-    s.off = s_off + threadIdx.x;
+    if constexpr (C::unrolled) {
+        s.off = s_off + threadIdx.x;
+    } else {
+        s.off = l_off;
+    }
     const float rec_h = 0.5f * p.muscle_recovery * C::h;
     const uint32_t waves = gridDim.x * (C::BLOCK / 64u);
     Body<C> b;
@@ -1209,12 +1221,30 @@ __global__ void __launch_bounds__(C::BLOCK) advance_kernel(const uint32_t* __res
             if constexpr (C::record) {
                 my_frames = frames + static_cast<uint64_t>(at) * frames_per_creature * p.stride;
             }
-            begin_creature<C>(b, tr, s, p.base + at, records, muscles, heads, p, flags, my_frames);
+            if constexpr (C::world == kTableWorld) {
+                tr.world = worlds.of[p.base + at];
+                begin_creature<C>(b, tr, s, p.base + at, records, muscles, heads, worlds.params[tr.world],
+                                  worlds.flags[tr.world], my_frames);
+            } else {
+                begin_creature<C>(b, tr, s, p.base + at, records, muscles, heads, p, flags, my_frames);
+            }
         }
         if (__ballot(tr.live) == 0ull) { break; }
         const Bounds wb{wave_max(tr.live ? b.nodes : 0u), wave_max(tr.live ? b.bones : 0u),
                         wave_max(tr.live ? b.muscles : 0u)};
-        if (tr.live) { run_step<C>(b, tr, s, muscles, results, p, flags, rec_h, my_frames, wb); }
+        if (tr.live) {
+            if constexpr (C::world == kTableWorld) {
+                // the lane's world: its parameters are read where they are
+                // used (L1-resident table), not held in registers
+                uint32_t w = tr.world;
+                opaque<C>(w);
+                const evo_params& pw = worlds.params[w];
+                const float rh = 0.5f * pw.muscle_recovery * C::h;
+                run_step<C>(b, tr, s, muscles, results, pw, worlds.flags[w], rh, my_frames, wb);
+            } else {
+                run_step<C>(b, tr, s, muscles, results, p, flags, rec_h, my_frames, wb);
+            }
+        }
     }
 }
 

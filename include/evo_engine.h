@@ -75,6 +75,35 @@ uint32_t evo_world_flags(const evo_world_config* c);
  * (`params`); run->record is left 0. */
 int evo_world_params(const evo_world_config* c, uint32_t base, uint32_t count, uint32_t stride, evo_params* p,
                      evo_run* run);
+/* The wild islands' worlds (src/environment.rs:479-542): EVO_EFFECTS effects
+ * in the reference's EFFECTS order (0 is the autochange level, 1 ground,
+ * 2 gravity, 3 air, 4 grip, 5 heat, 6 drought, 7 slope, 8 wind, 9 mud,
+ * 10 brambles, 11 water, 12 ice patches, 13 gaps, 14 hurdles, 15 quake). */
+#define EVO_EFFECTS 16u
+#define EVO_WILD_ISLANDS 100u
+typedef struct evo_wild_mix {
+    uint32_t count;     /* 1 to 3 effects */
+    uint32_t effect[3]; /* ascending effect indices */
+    uint32_t level[3];
+} evo_wild_mix;
+
+/* The first n (<= EVO_WILD_ISLANDS) wild islands' effect mixes, the same in
+ * every game (environment::wild_levels). */
+int evo_wild_levels(uint32_t n, evo_wild_mix* out);
+
+/* `base` with every effect calm except the island's mix, no screen and no
+ * rungs (environment::wild_world). */
+int evo_wild_world(const evo_world_config* base, const evo_wild_mix* mix, evo_world_config* out);
+
+/* n island worlds for the batched benchmark, island i from the reference's
+ * counter-based stream (src/evolution.rs:455-564) keyed by (seed, 0, i): ground
+ * bumps {0, 3, 8, 15, 25} cm, gravity {1, 1.5, 2, 3} g, grip U[0.05, 1.2],
+ * water {0, 12, 35, 90} cm, wind {0, -1, -3, -6} m/s^2; the rest of `base`. */
+int evo_island_worlds(uint64_t seed, uint32_t n, const evo_world_config* base, evo_world_config* out);
+
+/* The splitmix64 finalizer of that stream (src/evolution.rs:455-462). */
+uint64_t evo_splitmix_finalize(uint64_t z);
+
 /* Frames of the start pose before a recording's first step
  * (src/physics.rs:82-84, `Fidelity::settle`). */
 uint32_t evo_settle_frames(uint32_t rate);
@@ -112,7 +141,8 @@ typedef struct evo_synth_spec {
     float duty;
     float stiffness;
     float joint_lo, joint_hi; /* half ranges of the joints, radians */
-    uint32_t sorted;          /* pack sorted by (muscles, nodes, index) */
+    uint32_t sorted;          /* 1: pack sorted by (muscles, nodes, index); 2: the same within two body-size
+                                 classes, bodies of at most 12 nodes and 24 muscles first */
     uint32_t id_base;         /* creature c has id id_base + c (quake hash) */
     /* Ground the start pose stands on (world of creature c: worlds[world_of[c]]
      * when world_of is given to evo_synth_fill, else `world`). */
@@ -149,6 +179,60 @@ int evo_synth_fill(const evo_synth_spec* s, uint32_t* records, float* muscles, u
 int evo_advance_device(const uint32_t* d_records, const float* d_muscles, const uint32_t* d_heads,
                        evo_result* d_results, uint32_t* d_counter, const evo_params* p, const evo_run* run,
                        float* d_frames, void* stream);
+
+/* A batch whose creatures run in different worlds (the wild islands,
+ * src/environment.rs:479-536, run by the reference as one launch per world):
+ * world k < n_worlds has launch parameters d_worlds[k] (evo_world_params of
+ * its config, with this batch's count, base, steps, stride, screen and
+ * rungs) and effect bits d_world_flags[k]; packed creature c runs in world
+ * d_world_of[c]. run->flags is the union of the worlds' bits. `grid` is the
+ * number of blocks to launch, 0 for as many as fit on the GPU at once.
+ * Otherwise as evo_advance_device (no recording). */
+int evo_advance_worlds_device(const uint32_t* d_records, const float* d_muscles, const uint32_t* d_heads,
+                              evo_result* d_results, uint32_t* d_counter, const evo_params* p, const evo_run* run,
+                              const evo_params* d_worlds, const uint32_t* d_world_flags,
+                              const uint16_t* d_world_of, uint32_t n_worlds, int32_t grid, void* stream);
+
+/* Blocks of the kernel that this run would launch (table != 0: the
+ * per-creature world kernel) that fit on the GPU at once, or a negative
+ * EVO_ERR_* code. */
+int evo_kernel_capacity(const evo_run* run, int32_t table);
+
+/* One contiguous range of a packed batch and the kernel it runs: the largest
+ * node and muscle counts in it (which pick the body-size class), whether it
+ * is free of tendons, and its grid (blocks, 0 for as many as fit). */
+typedef struct evo_class_range {
+    uint32_t base;
+    uint32_t count;
+    uint32_t max_nodes;
+    uint32_t max_muscles;
+    uint32_t no_tendons;
+    int32_t grid;
+} evo_class_range;
+
+/* One generation as concurrent persistent kernels, one per range (at most
+ * 4), each claiming creatures of its range from its own counter
+ * d_counters[i]. The ranges fork from `stream` onto internal streams and
+ * join back into it, so work queued on `stream` afterwards sees every
+ * result; the call can be captured into a graph (evo_graph_begin). With
+ * d_worlds non-NULL every creature runs in its own world as in
+ * evo_advance_worlds_device. Replaces the reference's per-wave launches over
+ * up to 8 streams (src/cuda_engine.rs:2171-2311). */
+int evo_advance_ranges_device(const uint32_t* d_records, const float* d_muscles, const uint32_t* d_heads,
+                              evo_result* d_results, uint32_t* d_counters, const evo_params* p, const evo_run* run,
+                              const evo_class_range* ranges, uint32_t n_ranges, const evo_params* d_worlds,
+                              const uint32_t* d_world_flags, const uint16_t* d_world_of, uint32_t n_worlds,
+                              void* stream);
+
+/* HIP graphs of this library's launches: evo_graph_begin starts capturing
+ * `stream`, evo_graph_end stops and instantiates what was captured,
+ * evo_graph_launch replays it on a stream. The launches clear their claim
+ * counters with a fill kernel, so every replay starts from zero. */
+typedef struct evo_graph evo_graph;
+int evo_graph_begin(void* stream);
+int evo_graph_end(void* stream, evo_graph** out);
+int evo_graph_launch(evo_graph* graph, void* stream);
+void evo_graph_destroy(evo_graph* graph);
 
 /* Archive metrics and niche cells of creatures [base, base + count) from
  * their results, on the device: scheduler::to_metrics and trial_steps

@@ -38,6 +38,11 @@ def load() -> C.CDLL:
         lib.oracle_describe.restype = None
         lib.oracle_niche_in.argtypes = [C.c_uint32] + [C.c_float] * 5 + [C.c_uint32, C.c_void_p]
         lib.oracle_niche_in.restype = None
+        for name in ("d32_gradient", "d64_gradient"):
+            fn = getattr(lib, name)
+            fn.argtypes = [C.c_void_p] * 5 + [C.c_uint32, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p,
+                                              C.c_void_p, C.c_void_p]
+            fn.restype = C.c_int
         lib.oracle_cos_0_pi_n.argtypes = [C.c_void_p, C.c_void_p, C.c_long]
         lib.oracle_atan2_n.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_long]
         lib.oracle_f2h_n.argtypes = [C.c_void_p, C.c_void_p, C.c_long]
@@ -109,3 +114,48 @@ def niche_in(nodes: int, ground_contact: float, gait_frequency: float, aspect_ra
     cell = (C.c_uint8 * 6)()
     load().oracle_niche_in(nodes, ground_contact, gait_frequency, aspect_ratio, mean_height, feet, layout, cell)
     return tuple(cell)
+
+
+DIRS = 8
+
+
+def param_count(batch, c: int) -> int:
+    """Parameters of packed creature c's gradient: six per muscle (rhythm
+    amplitude, inverse period, phase, Hill factor, force cap, inverse energy
+    capacity), then one friction per node."""
+    h = batch.heads[c]
+    return 6 * int(h[1]) + int(h[0] & 255)
+
+
+def gradient(batch, params, run_, c: int, dirs: np.ndarray, precision: str = "f32"):
+    """Derivatives of creature c's fitness along the rows of `dirs` (at most
+    DIRS rows of param_count(batch, c) values) by forward-mode dual numbers:
+    (fitness, derivatives, steps). precision f32 follows the FP32 oracle's
+    trajectory bit for bit; f64 the FP64 oracle's."""
+    dirs = np.ascontiguousarray(np.atleast_2d(dirs), dtype=np.float64)
+    n = param_count(batch, c)
+    if dirs.shape[1] != n or dirs.shape[0] > DIRS:
+        raise ValueError(f"directions must be at most {DIRS} rows of {n} parameters")
+    fit = C.c_double(0.0)
+    der = np.zeros(dirs.shape[0], np.float64)
+    steps = C.c_long(0)
+    fn = load().d32_gradient if precision == "f32" else load().d64_gradient
+    rc = fn(batch.records.ctypes.data, batch.muscles.ctypes.data, batch.heads.ctypes.data, C.addressof(params),
+            C.addressof(run_), c, dirs.ctypes.data, dirs.shape[0], n, C.byref(fit), der.ctypes.data, C.byref(steps))
+    if rc != 0:
+        raise ValueError(f"oracle cannot differentiate creature {c}")
+    return fit.value, der, steps.value
+
+
+def full_gradient(batch, params, run_, c: int, precision: str = "f32"):
+    """The whole gradient of creature c's fitness, DIRS parameters a pass."""
+    n = param_count(batch, c)
+    out = np.zeros(n)
+    fit = None
+    for start in range(0, n, DIRS):
+        rows = min(DIRS, n - start)
+        dirs = np.zeros((rows, n))
+        dirs[np.arange(rows), start + np.arange(rows)] = 1.0
+        fit, der, _ = gradient(batch, params, run_, c, dirs, precision)
+        out[start:start + rows] = der
+    return fit, out

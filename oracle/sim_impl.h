@@ -58,6 +58,10 @@ typedef struct {
     R mstore[MAXM_], mdemand[MAXM_], moffset[MAXM_], mpull[MAXM_];
     uint32_t mnodes[MAXM_];
     R maa[MAXM_], mab[MAXM_], mhill[MAXM_], mcap[MAXM_], minvcap[MAXM_], mtk[MAXM_], mslack[MAXM_];
+    /* the rhythm's amplitude, inverse period and phase (muscle record words
+     * 8-10), held here so that a build with a derivative-carrying R can
+     * differentiate the trial with respect to them */
+    R mamp[MAXM_], minvp[MAXM_], mphase[MAXM_];
     R normal_force[MAXN_], friction_force[MAXN_];
     R px[MAXN_], py[MAXN_], vx[MAXN_], vy[MAXN_], fx[MAXN_], fy[MAXN_];
 } FN(Body);
@@ -215,7 +219,7 @@ static inline void FN(joint_push)(R* xs, R* ys, const FN(Joint)* k, R lambda) {
 static void FN(muscle_demands)(const FN(World)* w, FN(Body)* b, const float* mus, R t, int limp) {
     for (uint32_t k = 0u; k < b->muscles; k++) {
         const float* m = mus + k * EVO_MUSCLE_WORDS;
-        const R amplitude = m[8], inv_period = m[9], phase = m[10], duty = m[11];
+        const R amplitude = b->mamp[k], inv_period = b->minvp[k], phase = b->mphase[k], duty = m[11];
         const R inv_duty = m[12], inv_complement = m[13], stiffness = m[14];
         const R offset = b->moffset[k];
         const R before = FN(rhythm)(t, inv_period, phase, offset, duty, inv_duty, inv_complement);
@@ -560,6 +564,9 @@ static int FN(begin_creature)(const FN(World)* w, FN(Body)* b, FN(Trial)* tr, ui
         b->minvcap[k] = q[5];
         b->mtk[k] = q[6];
         b->mslack[k] = q[7];
+        b->mamp[k] = q[8];
+        b->minvp[k] = q[9];
+        b->mphase[k] = q[10];
     }
     tr->vmin = LIT(1e20);
     tr->vmax = LIT(-1e20);

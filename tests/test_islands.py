@@ -1,0 +1,165 @@
+"""Island worlds and the batched heterogeneous launch.
+
+CPU: the reference's random stream and its wild-island draw
+(src/evolution.rs:455-564, src/environment.rs:479-542), pinned by the
+published SplitMix64 sequence and by the rules the draw states. GPU: every
+creature of a class-major batch in its own world, as one launch, as
+concurrent per-class launches and as a replayed HIP graph, bit for bit
+against the FP32 oracle run world by world.
+"""
+import numpy as np
+import pytest
+
+import evolution_simulator_gpu_amd as E
+import oracle
+from evolution_simulator_gpu_amd import _native as N
+from tests.helpers import assert_bit_exact, bits
+
+# levels and calm level of EFFECTS[1..15] (src/environment.rs:216-383)
+LEVELS = {1: 5, 2: 4, 3: 4, 4: 5, 5: 4, 6: 4, 7: 5, 8: 4, 9: 4, 10: 4, 11: 4, 12: 4, 13: 4, 14: 4, 15: 4}
+CALM = {e: (1 if e == 4 else 0) for e in LEVELS}
+
+
+def test_stream_mixer_is_splitmix64():
+    """SplitMix64 from state 0 gives 0xe220a8397b1dcdaf, 0x6e789e6aa1b965f4,
+    0x06c45d188009454f (Vigna's splitmix64.c): the finalizer of the state
+    after k increments of the golden gamma."""
+    lib = N.lib()
+    gamma = 0x9E3779B97F4A7C15
+    want = [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
+    assert [lib.evo_splitmix_finalize((gamma * k) % 2**64) for k in (1, 2, 3)] == want
+
+
+def test_wild_levels_follow_the_draw_rules():
+    """src/environment.rs:479-521: 100 different mixes of one to three
+    distinct effects, sorted, none the autochange entry, each at a level
+    that is neither calm nor the harshest; the first n are a prefix."""
+    mixes = E.wild_levels()
+    assert len(mixes) == N.WILD_ISLANDS
+    assert len({tuple(m) for m in mixes}) == N.WILD_ISLANDS
+    for m in mixes:
+        assert 1 <= len(m) <= 3
+        effects = [e for e, _ in m]
+        assert effects == sorted(set(effects))
+        for e, level in m:
+            assert 1 <= e <= 15
+            assert level != CALM[e] and level < LEVELS[e] - 1
+    counts = np.bincount([len(m) for m in mixes], minlength=4)[1:]
+    assert counts.min() > 15  # one, two and three effects all drawn
+    assert E.wild_levels(7) == mixes[:7]
+
+
+def test_wild_world_sets_only_its_mix():
+    base = E.world(terrain=0.08, gravity=19.6, ground_friction=0.6, screen_seconds=5.0, screen_bar=0.1)
+    calm = E.wild_world(base, [])
+    assert (calm.terrain, calm.gravity, calm.ground_friction, calm.screen_seconds) == (0.0, np.float32(9.8), 1.5, 0.0)
+    w = E.wild_world(base, [(2, 2), (8, 1), (11, 1)])
+    assert w.gravity == np.float32(19.6) and w.wind == -1.0 and w.water == np.float32(0.12)
+    assert w.terrain == 0.0 and w.mud == 0.0 and w.ground_friction == 1.5
+    assert np.isinf(w.r1.bias) and w.r1.bias < 0
+
+
+def test_island_worlds_draw_from_their_tables():
+    ws = E.island_worlds(200, seed=3)
+    t = {np.float32(v) for v in (0.0, 0.03, 0.08, 0.15, 0.25)}
+    assert {np.float32(w.terrain) for w in ws} == t
+    assert {np.float32(w.gravity) for w in ws} == {np.float32(v) for v in (9.8, 14.7, 19.6, 29.4)}
+    assert {np.float32(w.water) for w in ws} == {np.float32(v) for v in (0.0, 0.12, 0.35, 0.9)}
+    assert {w.wind for w in ws} == {0.0, -1.0, -3.0, -6.0}
+    grip = np.array([w.ground_friction for w in ws])
+    assert grip.min() >= 0.05 and grip.max() < 1.2 and grip.std() > 0.25
+    again = E.island_worlds(200, seed=3)
+    assert all(bytes(a) == bytes(b) for a, b in zip(ws, again))
+    other = E.island_worlds(200, seed=4)
+    assert any(bytes(a) != bytes(b) for a, b in zip(ws, other))
+
+
+def _island_batch(count=1500, n_worlds=16, seed=5, duration=2.0):
+    """A class-major batch spread over island worlds and wild worlds (which
+    switch on every other effect)."""
+    base = E.world(duration=duration, substeps=8)
+    worlds = E.island_worlds(n_worlds // 2, seed=seed, base=base)
+    worlds += [E.wild_world(base, m) for m in E.wild_levels(n_worlds - len(worlds))]
+    spec = E.synth_spec(2, count)
+    spec.sorted = 2
+    rng = np.random.default_rng(seed)
+    world_of_gen = rng.integers(0, len(worlds), count).astype(np.uint32)
+    batch = E.synth(spec, worlds=worlds, world_of=world_of_gen)
+    world_of = world_of_gen[batch.slots]
+    table = E.WorldTable(worlds, world_of, batch.count, E.frame_stride(batch))
+    return batch, worlds, world_of, table
+
+
+def _oracle_by_world(batch, worlds, world_of):
+    want = np.zeros(batch.count, N.RESULT_DTYPE)
+    for k, w in enumerate(worlds):
+        idx = np.nonzero(world_of == k)[0]
+        if idx.size == 0:
+            continue
+        sub = batch.subset(idx)
+        p, run = E.run_for(sub, w)
+        res, _, _ = oracle.run_f32(sub, p, run, threads=8)
+        want[idx] = res
+    return want
+
+
+def test_world_table_host_checks():
+    batch, worlds, world_of, table = _island_batch(count=64, n_worlds=4)
+    with pytest.raises(ValueError):
+        E.WorldTable(worlds, world_of[:-1], batch.count, 0)
+    with pytest.raises(ValueError):
+        E.WorldTable(worlds, np.full(batch.count, 9), batch.count, 0)
+    with pytest.raises(ValueError):
+        E.WorldTable([worlds[0], E.world(duration=3.0, substeps=8)], world_of % 2, batch.count, 0)
+    assert table.flag_union & N.TERRAIN and table.flag_union & N.GROUND
+    ranges = E.class_ranges(batch)
+    assert [r[0] for r in ranges] == [0, ranges[0][1]] and sum(r[1] for r in ranges) == batch.count
+    assert ranges[0][2] <= 12 and ranges[0][3] <= 24
+
+
+@pytest.mark.gpu
+def test_every_creature_runs_in_its_own_world(gpu):
+    import torch
+
+    batch, worlds, world_of, table = _island_batch()
+    want = _oracle_by_world(batch, worlds, world_of)
+    p, run = table.run_for(batch)
+    db = E.DeviceBatch(batch)
+    db.set_worlds(table)
+    db.launch_worlds(p, run)
+    torch.cuda.synchronize()
+    one = db.results_numpy()
+    assert_bit_exact(one, want, "one launch, per-creature worlds")
+    db.results.zero_()
+    db.launch_ranges(p, run, E.class_ranges(batch))
+    torch.cuda.synchronize()
+    assert np.array_equal(bits(db.results_numpy()), bits(one))
+    # different worlds give different trials of the same body plans
+    assert len({round(float(x), 3) for x in one["fitness"]}) > batch.count // 2
+
+
+@pytest.mark.gpu
+def test_replayed_generation_graph_matches_eager(gpu):
+    """The concurrent per-class launches captured once and replayed three
+    times, with eager work between the replays: every replay equals the
+    eager run bit for bit."""
+    import torch
+
+    batch, worlds, world_of, table = _island_batch(count=900, n_worlds=10, seed=9, duration=1.0)
+    p, run = table.run_for(batch)
+    db = E.DeviceBatch(batch)
+    db.set_worlds(table)
+    ranges = E.class_ranges(batch)
+    side = torch.cuda.Stream()
+    with torch.cuda.stream(side):
+        db.launch_ranges(p, run, ranges, stream=side)
+    torch.cuda.synchronize()
+    eager = db.results_numpy()
+    g = E.Graph()
+    g.capture(lambda s: db.launch_ranges(p, run, ranges, stream=s), side)
+    for rep in range(3):
+        db.results.fill_(float(rep + 1))
+        torch.cuda.synchronize()
+        g.replay(side)
+        torch.cuda.synchronize()
+        assert np.array_equal(bits(db.results_numpy()), bits(eager)), f"replay {rep}"
