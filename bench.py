@@ -52,7 +52,7 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--replays", type=int, default=64)
-    ap.add_argument("--workload", choices=["forward", "islands"], default="forward",
+    ap.add_argument("--workload", choices=["forward", "islands", "adjoint"], default="forward",
                     help="forward: 3M creatures, flat ground (default); islands: 3M creatures of 3-32 nodes over "
                          "100 island worlds, batched vs per-shape launches")
     ap.add_argument("--islands", type=int, default=100)
@@ -105,7 +105,7 @@ def islands(args, torch):
     note(f"generated {batch.count} creatures in {time.perf_counter() - t0:.1f} s")
 
     gen = IslandGeneration(batch, worlds, world_of)
-    tuning = gen.tune(seconds=2.0)
+    tuning = gen.tune(seconds=2.0, progress=note)
     note(f"tuned: {tuning}")
     gen.capture()
     for _ in range(args.warmup):
@@ -172,6 +172,109 @@ def islands(args, torch):
     print(json.dumps(line))
 
 
+def adjoint(args, torch):
+    """65,536 creatures (4-10 nodes, 6-16 muscles), 10 s trials on flat and
+    on 8 cm rough ground: the plain forward launch against forward + backward
+    (checkpoints every 60 steps), and the gradient checked on a sample."""
+    import oracle  # the checker of the sampled gradients
+
+    import evolution_simulator_gpu_amd as E
+    from evolution_simulator_gpu_amd import adjoint as A
+
+    creatures = args.creatures if args.creatures != 3_000_000 else 65_536
+    duration = args.duration if args.duration != 20.0 else 10.0
+    batch = E.synth(E.synth_spec(3, creatures))
+    batch.validate()
+    worlds = {"flat": E.world(duration=duration, substeps=8), "rough 8 cm": E.world(duration=duration, substeps=8,
+                                                                                     terrain=0.08)}
+    out = {}
+    rng = np.random.default_rng(11)
+    for name, w in worlds.items():
+        p, run = E.run_for(batch, w)
+        pair = A.Pair(E.DeviceBatch(batch), p, run, seg=60)
+        for _ in range(args.warmup):
+            pair.forward()
+            pair.backward()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            pair.forward()
+        torch.cuda.synchronize()
+        fwd = (time.perf_counter() - t0) / args.steps
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            pair.forward()
+            pair.backward()
+        torch.cuda.synchronize()
+        both = (time.perf_counter() - t0) / args.steps
+        res = pair.db.results_numpy()
+        same = bool(np.array_equal(pair.fitness.cpu().numpy().view(np.uint32), res["fitness"].view(np.uint32)))
+        g = pair.gradient_numpy()
+        substeps = int((res["ground_hi"].view(np.uint32) >> 16).astype(np.int64).sum()) * 8
+        note(f"{name}: forward {fwd * 1e3:.1f} ms, forward + backward {both * 1e3:.1f} ms")
+        # the FP32 trajectory's own derivative (dual oracle) along one random
+        # direction per sampled creature
+        sample = rng.choice(batch.count, min(256, batch.count), replace=False)
+        dirs = [rng.normal(size=oracle.param_count(batch, int(c))) for c in sample]
+        _, d32 = oracle.directional(batch, p, run, sample, dirs, "f32", threads=16)
+        mine = np.array([g[c, :len(d)] @ d for c, d in zip(sample, dirs)])
+        rel32 = np.abs(mine - d32) / np.maximum(np.abs(d32), 1e-12)
+        # central differences of the FP64 oracle in gene space, h = 1e-3
+        base = A.rebuilt(batch, sample)
+        gdirs = [A.gene_direction(A.genes_of(base, i), rng) for i in range(base.count)]
+        up, down = A.moved(base, gdirs, 1e-3), A.moved(base, gdirs, -1e-3)
+        zeros = [np.zeros(oracle.param_count(base, i)) for i in range(base.count)]
+        f_up, _ = oracle.directional(up, p, run, range(base.count), zeros, "f64", threads=16)
+        f_dn, _ = oracle.directional(down, p, run, range(base.count), zeros, "f64", threads=16)
+        fd = (f_up - f_dn) / 2e-3
+        pb, rb = E.run_for(base, w)
+        rb.flags = run.flags
+        bpair = A.Pair(E.DeviceBatch(base), pb, rb, seg=60)
+        bpair.forward()
+        bpair.backward()
+        torch.cuda.synchronize()
+        bg = bpair.gradient_numpy()
+        adj = np.array([sum(float(A.gene_gradient(base, i, bg[i])[k] @ gdirs[i][k]) for k in A.GENES)
+                        for i in range(base.count)])
+        # the FP64 trajectory's own derivative along the same directions
+        pdirs = [A.param_direction(base, i, gdirs[i]) for i in range(base.count)]
+        _, dd64 = oracle.directional(base, p, run, range(base.count), pdirs, "f64", threads=16)
+        rel_fd = np.abs(adj - fd) / np.maximum(np.abs(fd), 1e-12)
+        rel_ad64 = np.abs(dd64 - fd) / np.maximum(np.abs(fd), 1e-12)
+        out[name] = {
+            "forward_ms": fwd * 1e3, "forward_backward_ms": both * 1e3, "ratio": both / fwd,
+            "creature_substeps": substeps, "fitness_bit_exact": same,
+            "vs_fp32_dual_oracle": {"median_rel": float(np.median(rel32)), "p99_rel": float(np.percentile(rel32, 99)),
+                                    "within_1e-3": float((rel32 < 1e-3).mean())},
+            "vs_fp64_central_differences_h1e-3": {
+                "median_rel": float(np.median(rel_fd)), "within_1e-3": float((rel_fd < 1e-3).mean()),
+                "within_1e-2": float((rel_fd < 1e-2).mean()),
+                "fp64_derivative_within_1e-3_of_differences": float((rel_ad64 < 1e-3).mean())},
+        }
+    ratio = max(v["ratio"] for v in out.values())
+    line = {
+        "metric": "fwd+adjoint/fwd cost ratio at 65,536 creatures, 10 s trials",
+        "value": ratio,
+        "unit": "x",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": max(v["forward_backward_ms"] for v in out.values()),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "adjoint: 65,536 creatures from seed 4, 4-10 nodes, 6-16 muscles, 10 s trials at "
+                               "60 Hz x 8 substeps, flat and 8 cm rough ground, checkpoints every 60 steps; "
+                               "d(fitness)/d(period, phase, contracted, extended, strength, friction)",
+                   "creatures": batch.count, "worlds": out},
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -192,6 +295,8 @@ def main():
 
     if args.workload == "islands":
         return islands(args, torch)
+    if args.workload == "adjoint":
+        return adjoint(args, torch)
 
     import evolution_simulator_gpu_amd as E
     from evolution_simulator_gpu_amd import _native as N

@@ -148,6 +148,8 @@ SIGNATURES = {
     "evo_wild_world": ([C.POINTER(WorldConfig), C.POINTER(WildMix), C.POINTER(WorldConfig)], C.c_int),
     "evo_island_worlds": ([_u64, _u32, C.POINTER(WorldConfig), _P], C.c_int),
     "evo_splitmix_finalize": ([_u64], _u64),
+    "evo_adjoint_workspace_bytes": ([_u32, _u32, _u32, _u32], _u64),
+    "evo_adjoint_device": ([_P, _P, _P, _P, C.POINTER(Params), C.POINTER(Run), _u32, _P, _P, _P, _u64, _P], C.c_int),
     "evo_validate_batch": ([_P, _u64, _P, _u64, _P, _u32, C.POINTER(_u32)], C.c_int),
     "evo_max_muscles": ([_P, _u32, _u32], _u32),
     "evo_build_info": ([], C.c_char_p),

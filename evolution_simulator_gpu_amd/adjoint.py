@@ -1,0 +1,211 @@
+"""Gradients of trial fitness: the forward/backward pair.
+
+`forward` is the plain simulator launch; `backward` the adjoint launch
+(evo_adjoint_device), which recomputes each creature's trajectory from
+checkpoints every `seg` steps and returns d(fitness)/d(record parameters):
+per muscle the rhythm amplitude, inverse period and phase, the Hill factor,
+the force cap and the inverse energy capacity; per node the friction.
+
+`gene_gradient` carries that gradient to the genes a body is built from
+(src/physics2.rs:233-262, src/kernel.rs:376-408, as the synthetic generator
+builds its records): per muscle the rhythm period and phase, the contracted
+and extended lengths and the strength; per node the friction. `ascend`
+moves elites' genes along it, the gradient-directed mutation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _native as N
+from .batch import Batch, DeviceBatch, concat
+
+STRIDE = 6 * 24 + 12
+MAX_NODES, MAX_MUSCLES = 12, 24
+PER_MUSCLE = ("amplitude", "inv_period", "phase", "hill", "cap", "inv_capacity")
+DUTY_SPEED = 24.0  # the rhythm's speed limit behind the amplitude cap (m/s)
+
+
+class Pair:
+    """Forward and backward launches over one device-resident batch."""
+
+    def __init__(self, db: DeviceBatch, p: N.Params, run: N.Run, seg: int = 60):
+        import torch
+
+        if run.max_nodes > MAX_NODES or run.max_muscles > MAX_MUSCLES:
+            raise ValueError("the adjoint handles bodies of at most 12 nodes and 24 muscles")
+        self.db, self.p, self.run, self.seg = db, p, run, seg
+        n = db.batch.count
+        size = N.lib().evo_adjoint_workspace_bytes(n, p.steps, run.substeps, seg)
+        self.workspace = torch.empty(size, dtype=torch.uint8, device=db.device)
+        self.grads = torch.zeros(n * STRIDE, dtype=torch.float32, device=db.device)
+        self.fitness = torch.zeros(n, dtype=torch.float32, device=db.device)
+
+    def forward(self, stream=None) -> None:
+        self.db.launch(self.p, self.run, stream)
+
+    def backward(self, stream=None) -> None:
+        import torch
+
+        if stream is None:
+            stream = torch.cuda.current_stream(self.db.records.device)
+        rc = N.lib().evo_adjoint_device(
+            self.db.records.data_ptr(), self.db.muscles.data_ptr(), self.db.heads.data_ptr(),
+            self.db.results.data_ptr(), C.byref(self.p), C.byref(self.run), self.seg, self.grads.data_ptr(),
+            self.fitness.data_ptr(), self.workspace.data_ptr(), self.workspace.numel(),
+            C.c_void_p(stream.cuda_stream))
+        if rc != N.EVO_OK:
+            raise RuntimeError(f"evo_adjoint_device failed: {rc}")
+
+    def gradient_numpy(self) -> np.ndarray:
+        return self.grads.cpu().numpy().reshape(-1, STRIDE)
+
+
+def params_of(batch: Batch, c: int) -> np.ndarray:
+    """Creature c's record parameters in gradient order."""
+    h = batch.heads[c]
+    n, m = int(h[0] & 255), int(h[1])
+    mus = batch.muscles[int(h[4]):int(h[4]) + m * N.MUSCLE_WORDS].reshape(m, N.MUSCLE_WORDS)
+    nodes = batch.records[int(h[3]):int(h[3]) + n * N.NODE_WORDS].reshape(n, N.NODE_WORDS)
+    out = np.zeros(6 * m + n, np.float64)
+    out[:6 * m] = mus[:, [8, 9, 10, 3, 4, 5]].reshape(-1)
+    out[6 * m:] = nodes[:, 2].view(np.float32)
+    return out
+
+
+def genes_of(batch: Batch, c: int) -> dict:
+    """The genes behind creature c's records: per muscle period, phase,
+    contracted and extended length, strength and duty; per node friction."""
+    h = batch.heads[c]
+    n, m = int(h[0] & 255), int(h[1])
+    mus = batch.muscles[int(h[4]):int(h[4]) + m * N.MUSCLE_WORDS].reshape(m, N.MUSCLE_WORDS).astype(np.float64)
+    nodes = batch.records[int(h[3]):int(h[3]) + n * N.NODE_WORDS].reshape(n, N.NODE_WORDS)
+    period = 1.0 / mus[:, 9]
+    extended = 1.0 / (8.0 * mus[:, 3])
+    return {
+        "period": period, "phase": mus[:, 10].copy(), "extended": extended,
+        "contracted": extended - mus[:, 8], "strength": mus[:, 4] / 200.0, "duty": mus[:, 11].copy(),
+        "friction": nodes[:, 2].view(np.float32).astype(np.float64),
+    }
+
+
+def records_from_genes(batch: Batch, c: int, genes: dict) -> Batch:
+    """A copy of the batch with creature c's parameter words rebuilt from
+    `genes` as the generator builds them (float32 rounding included)."""
+    out = Batch(batch.records.copy(), batch.muscles.copy(), batch.heads.copy(), batch.slots.copy())
+    h = out.heads[c]
+    n, m = int(h[0] & 255), int(h[1])
+    mus = out.muscles[int(h[4]):int(h[4]) + m * N.MUSCLE_WORDS].reshape(m, N.MUSCLE_WORDS)
+    f32 = np.float32
+    for k in range(m):
+        period, duty = f32(genes["period"][k]), f32(genes["duty"][k])
+        longest, shortest = f32(genes["extended"][k]), f32(genes["contracted"][k])
+        strength = f32(genes["strength"][k])
+        cap_amp = f32(f32(f32(f32(2.0) * f32(DUTY_SPEED)) * period) * f32(min(duty, f32(1.0) - duty))) / f32(np.pi)
+        mus[k, 8] = min(f32(longest - shortest), f32(cap_amp))
+        mus[k, 9] = f32(1.0) / period
+        mus[k, 10] = f32(genes["phase"][k])
+        mus[k, 3] = f32(1.0) / (f32(8.0) * max(longest, f32(0.05)))
+        mus[k, 4] = f32(200.0) * strength
+        mus[k, 5] = f32(1.0) / (f32(120.0) * strength)
+    nodes = out.records[int(h[3]):int(h[3]) + n * N.NODE_WORDS].reshape(n, N.NODE_WORDS)
+    nodes[:, 2] = np.asarray(genes["friction"], np.float32).view(np.uint32)
+    return out
+
+
+def gene_gradient(batch: Batch, c: int, grad: np.ndarray) -> dict:
+    """d(fitness)/d(genes) of creature c from its record gradient (one row
+    of the adjoint's output) by the chain rule through records_from_genes."""
+    h = batch.heads[c]
+    n, m = int(h[0] & 255), int(h[1])
+    g = np.asarray(grad, np.float64)[:6 * m + n]
+    gm = g[:6 * m].reshape(m, 6)
+    genes = genes_of(batch, c)
+    period, duty = genes["period"], genes["duty"]
+    longest, strength = genes["extended"], genes["strength"]
+    cap_amp = 2.0 * DUTY_SPEED * period * np.minimum(duty, 1.0 - duty) / np.pi
+    capped = (longest - genes["contracted"]) > cap_amp
+    g_amp, g_invp, g_phase, g_hill, g_cap, g_invcap = (gm[:, i] for i in range(6))
+    return {
+        "period": g_invp * (-1.0 / period ** 2) + np.where(capped, g_amp * cap_amp / period, 0.0),
+        "phase": g_phase,
+        "contracted": np.where(capped, 0.0, -g_amp),
+        "extended": np.where(capped, 0.0, g_amp)
+        + np.where(longest > 0.05, g_hill * (-1.0 / (8.0 * longest ** 2)), 0.0),
+        "strength": g_cap * 200.0 + g_invcap * (-1.0 / (120.0 * strength ** 2)),
+        "friction": g[6 * m:],
+    }
+
+
+def param_direction(batch: Batch, c: int, gene_dir: dict) -> np.ndarray:
+    """The record-parameter direction that a gene direction moves creature
+    c along (the Jacobian of records_from_genes times the direction), in
+    gradient order: gene_gradient(g) . d == g . param_direction(d)."""
+    h = batch.heads[c]
+    m = int(h[1])
+    genes = genes_of(batch, c)
+    period, duty = genes["period"], genes["duty"]
+    longest, strength = genes["extended"], genes["strength"]
+    cap_amp = 2.0 * DUTY_SPEED * period * np.minimum(duty, 1.0 - duty) / np.pi
+    capped = (longest - genes["contracted"]) > cap_amp
+    d = gene_dir
+    out = np.zeros((m, 6))
+    out[:, 0] = np.where(capped, cap_amp / period * d["period"], d["extended"] - d["contracted"])
+    out[:, 1] = -d["period"] / period ** 2
+    out[:, 2] = d["phase"]
+    out[:, 3] = np.where(longest > 0.05, -d["extended"] / (8.0 * longest ** 2), 0.0)
+    out[:, 4] = 200.0 * d["strength"]
+    out[:, 5] = -d["strength"] / (120.0 * strength ** 2)
+    return np.concatenate([out.reshape(-1), np.asarray(d["friction"], np.float64)])
+
+
+def ascend(batch: Batch, creatures, grads: np.ndarray, rate: float = 0.02) -> Batch:
+    """Gradient-directed mutation: each listed creature's genes take a step
+    of relative size `rate` along its fitness gradient (scaled per gene by
+    the gene's own magnitude), kept inside the generator's ranges. Returns a
+    new batch; the search operators that pick elites are the caller's."""
+    out = batch
+    for row, c in enumerate(creatures):
+        genes = genes_of(out, c)
+        gg = gene_gradient(out, c, grads[row])
+        for key in ("period", "phase", "contracted", "extended", "strength", "friction"):
+            scale = np.maximum(np.abs(genes[key]), 1e-3)
+            d = gg[key] * scale
+            norm = np.abs(d).max()
+            if norm > 0.0:
+                genes[key] = genes[key] + rate * scale * d / norm
+        genes["period"] = np.clip(genes["period"], 0.5, 2.0)
+        genes["phase"] = np.mod(genes["phase"], 1.0)
+        genes["strength"] = np.clip(genes["strength"], 0.1, 1.0)
+        genes["friction"] = np.clip(genes["friction"], 0.2, 1.0)
+        genes["contracted"] = np.minimum(genes["contracted"], genes["extended"])
+        out = records_from_genes(out, c, genes)
+    return out
+
+
+GENES = ("period", "phase", "contracted", "extended", "strength", "friction")
+
+
+def gene_direction(genes: dict, rng) -> dict:
+    """A random direction in gene space, each gene's component scaled by the
+    gene's magnitude, normalised to unit length."""
+    d = {k: rng.normal(size=len(genes[k])) * np.maximum(np.abs(genes[k]), 1e-2) for k in GENES}
+    norm = np.sqrt(sum(float((v ** 2).sum()) for v in d.values()))
+    return {k: v / norm for k, v in d.items()}
+
+
+def rebuilt(batch: Batch, creatures) -> Batch:
+    """The listed creatures as a batch whose parameter words are rebuilt from
+    their genes, so that gene perturbations start from a consistent point."""
+    return concat([records_from_genes(batch.subset([c]), 0, genes_of(batch, c)) for c in creatures])
+
+
+def moved(base: Batch, directions: list, h: float) -> Batch:
+    """base with creature i's genes moved by h * directions[i]."""
+    out = []
+    for i, d in enumerate(directions):
+        one = base.subset([i])
+        g = genes_of(one, 0)
+        out.append(records_from_genes(one, 0, {**g, **{k: g[k] + h * d[k] for k in GENES}}))
+    return concat(out)

@@ -380,6 +380,23 @@ def simulate(batch: Batch, w: N.WorldConfig, record: bool = False, device: str =
     return res
 
 
+def concat(batches: list[Batch]) -> Batch:
+    """One batch holding the creatures of `batches` in order."""
+    recs, muss, heads, slots = [], [], [], []
+    rw = mf = 0
+    for b in batches:
+        h = b.heads.copy()
+        h[:, 3] += rw
+        h[:, 4] += mf
+        recs.append(b.records)
+        muss.append(b.muscles)
+        heads.append(h)
+        slots.append(b.slots)
+        rw += b.records.size
+        mf += b.muscles.size
+    return Batch(np.concatenate(recs), np.concatenate(muss), np.concatenate(heads), np.concatenate(slots))
+
+
 def _f32_to_f16(v: float) -> int:
     return int(np.array([v], dtype=np.float32).astype(np.float16).view(np.uint16)[0])
 

@@ -62,7 +62,7 @@ class IslandGeneration:
     def launch(self, grids=None, p: N.Params | None = None) -> None:
         self.db.launch_ranges(p or self.p, self.run, self._ranges(grids or self.grids), stream=self.stream)
 
-    def tune(self, seconds: float = 2.0, candidates=None) -> dict:
+    def tune(self, seconds: float = 2.0, candidates=None, progress=None) -> dict:
         """Times each candidate split of the GPU between the classes on this
         batch with trials cut to `seconds`, keeps the fastest, and returns
         the timings. A candidate is one grid (blocks) per class; 0 means as
@@ -95,6 +95,8 @@ class IslandGeneration:
                 e1.record(self.stream)
                 torch.cuda.synchronize()
                 timings[tuple(int(x) for x in cand)] = e0.elapsed_time(e1)
+                if progress is not None:
+                    progress(f"grid split {tuple(int(x) for x in cand)}: {timings[tuple(int(x) for x in cand)]:.0f} ms")
         finally:
             torch.cuda.synchronize()
             self.db.set_worlds(self.table)

@@ -234,6 +234,36 @@ int evo_graph_end(void* stream, evo_graph** out);
 int evo_graph_launch(evo_graph* graph, void* stream);
 void evo_graph_destroy(evo_graph* graph);
 
+/* ---------------------------------------------------------------- gradients */
+
+/* Bodies the adjoint handles, and the values per creature of its gradient:
+ * per muscle k six at 6k .. 6k+5 (rhythm amplitude, inverse period, phase,
+ * Hill factor, force cap, inverse energy capacity: muscle record words 8, 9,
+ * 10, 3, 4, 5), then per node i its friction (node record word 2) at
+ * 6M + i; zeros past a creature's own parameters. */
+#define EVO_ADJ_MAX_NODES 12
+#define EVO_ADJ_MAX_MUSCLES 24
+#define EVO_ADJ_STRIDE (6u * EVO_ADJ_MAX_MUSCLES + EVO_ADJ_MAX_NODES)
+
+/* Device workspace of an adjoint launch: a checkpoint every `seg` steps and
+ * the start state of every substep of one segment, per creature. */
+uint64_t evo_adjoint_workspace_bytes(uint32_t count, uint32_t steps, uint32_t substeps, uint32_t seg);
+
+/* The backward half of a forward/backward pair (no counterpart in the
+ * reference, which has no gradients): given the results of
+ * evo_advance_device on the same batch and run, writes for creatures
+ * [p->base, p->base + p->count) d(fitness)/d(parameters) to
+ * d_grads[(c - base) * EVO_ADJ_STRIDE ...] and the fitness that the
+ * recomputed trajectory reaches (equal to the forward result) to
+ * d_fitness[c - base]. The trajectory is recomputed from checkpoints every
+ * `seg` steps. Worlds: ground, terrain, quake, slope, gaps, hurdles, ice,
+ * wind and air (EVO_ERR_INVALID for mud, water and brambles); bodies of at
+ * most EVO_ADJ_MAX_NODES nodes and EVO_ADJ_MAX_MUSCLES muscles
+ * (run->max_nodes, run->max_muscles). A failed trial has a zero gradient. */
+int evo_adjoint_device(const uint32_t* d_records, const float* d_muscles, const uint32_t* d_heads,
+                       const evo_result* d_results, const evo_params* p, const evo_run* run, uint32_t seg,
+                       float* d_grads, float* d_fitness, void* d_workspace, uint64_t workspace_bytes, void* stream);
+
 /* Archive metrics and niche cells of creatures [base, base + count) from
  * their results, on the device: scheduler::to_metrics and trial_steps
  * (src/scheduler.rs:859-923), then qd::descriptor and Descriptor::niche_in

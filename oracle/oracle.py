@@ -43,6 +43,11 @@ def load() -> C.CDLL:
             fn.argtypes = [C.c_void_p] * 5 + [C.c_uint32, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p,
                                               C.c_void_p, C.c_void_p]
             fn.restype = C.c_int
+        for name in ("d32_gradient_batch", "d64_gradient_batch"):
+            fn = getattr(lib, name)
+            fn.argtypes = [C.c_void_p] * 5 + [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                                              C.c_void_p]
+            fn.restype = C.c_int
         lib.oracle_cos_0_pi_n.argtypes = [C.c_void_p, C.c_void_p, C.c_long]
         lib.oracle_atan2_n.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_long]
         lib.oracle_f2h_n.argtypes = [C.c_void_p, C.c_void_p, C.c_long]
@@ -159,3 +164,23 @@ def full_gradient(batch, params, run_, c: int, precision: str = "f32"):
         fit, der, _ = gradient(batch, params, run_, c, dirs, precision)
         out[start:start + rows] = der
     return fit, out
+
+
+def directional(batch, params, run_, creatures, dirs, precision: str = "f64", threads: int = 8):
+    """Fitness and one directional derivative per listed creature (dirs[i]:
+    that creature's param_count components), on host threads."""
+    creatures = np.ascontiguousarray(creatures, dtype=np.uint32)
+    flat = np.ascontiguousarray(np.concatenate([np.asarray(d, np.float64) for d in dirs]))
+    sizes = np.array([param_count(batch, int(c)) for c in creatures], np.int64)
+    if any(len(d) != s for d, s in zip(dirs, sizes)):
+        raise ValueError("one direction of param_count values per creature")
+    offsets = np.ascontiguousarray(np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.uint32))
+    fit = np.zeros(len(creatures), np.float64)
+    der = np.zeros(len(creatures), np.float64)
+    fn = load().d32_gradient_batch if precision == "f32" else load().d64_gradient_batch
+    rc = fn(batch.records.ctypes.data, batch.muscles.ctypes.data, batch.heads.ctypes.data, C.addressof(params),
+            C.addressof(run_), creatures.ctypes.data, len(creatures), flat.ctypes.data, offsets.ctypes.data, threads,
+            fit.ctypes.data, der.ctypes.data)
+    if rc != 0:
+        raise ValueError("oracle cannot differentiate a listed creature")
+    return fit, der

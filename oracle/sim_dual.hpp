@@ -13,6 +13,8 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <atomic>
+#include <thread>
 #include <type_traits>
 #include <vector>
 
@@ -177,6 +179,34 @@ int FN(gradient)(const uint32_t* records, const float* muscles, const uint32_t* 
     for (uint32_t j = 0; j < ndirs; j++) { derivs[j] = tr->fitness.d[j]; }
     if (steps_out) { *steps_out = steps; }
     return 0;
+}
+
+// The trials of creatures list[0..n) with one derivative direction each
+// (dirs holds each creature's nparams[i] components one after the other),
+// on `threads` host threads: fitness[i] and derivs[i]. Returns 0, or -1 if a
+// creature cannot be differentiated.
+int FN(gradient_batch)(const uint32_t* records, const float* muscles, const uint32_t* heads, const evo_params* p,
+                       const evo_run* run, const uint32_t* list, uint32_t n, const double* dirs,
+                       const uint32_t* offsets, int threads, double* fitness, double* derivs) {
+    std::atomic<uint32_t> next{0};
+    std::atomic<int> bad{0};
+    auto work = [&]() {
+        for (;;) {
+            const uint32_t i = next.fetch_add(1);
+            if (i >= n) { break; }
+            const uint32_t c = list[i];
+            const uint32_t np_ = 6u * heads[8u * c + 1u] + (heads[8u * c] & 255u);
+            if (FN(gradient)(records, muscles, heads, p, run, c, dirs + offsets[i], 1u, np_, fitness + i, derivs + i,
+                             nullptr) != 0) {
+                bad = 1;
+            }
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < threads; t++) { pool.emplace_back(work); }
+    work();
+    for (auto& t : pool) { t.join(); }
+    return bad ? -1 : 0;
 }
 
 }  // extern "C"

@@ -1,0 +1,172 @@
+"""Gradients of trial fitness.
+
+The reference has no gradients; the adjoint differentiates its trial
+physics. Pinning: the dual-number oracle (oracle/sim_dual.hpp, forward-mode
+derivatives of the oracle's own trajectory) against central differences of
+the FP64 oracle on smooth short trials; the gene-level chain rule against
+central differences through the record builder; then the HIP adjoint against
+the dual oracle on the same FP32 trajectory, and the recomputed fitness bit
+for bit against the forward launch.
+"""
+import numpy as np
+import pytest
+
+import evolution_simulator_gpu_amd as E
+import oracle
+from evolution_simulator_gpu_amd import adjoint as A
+from evolution_simulator_gpu_amd import _native as N
+
+WORDS = [8, 9, 10, 3, 4, 5]  # muscle record words in gradient order
+
+
+def _fitness64(batch, p, run, c):
+    n = oracle.param_count(batch, c)
+    fit, _, _ = oracle.gradient(batch, p, run, c, np.zeros((1, n)), "f64")
+    return fit
+
+
+def _perturbed(batch, c, q, delta):
+    """A copy of the batch with creature c's parameter q moved by about
+    `delta` (relative); returns the copy and the exact float32 change."""
+    b = E.Batch(batch.records.copy(), batch.muscles.copy(), batch.heads.copy(), batch.slots.copy())
+    h = b.heads[c]
+    m = int(h[1])
+    if q < 6 * m:
+        k, f = divmod(q, 6)
+        idx = int(h[4]) + k * N.MUSCLE_WORDS + WORDS[f]
+        base = float(b.muscles[idx])
+        b.muscles[idx] = np.float32(base + delta * max(abs(base), 1e-2))
+        return b, float(b.muscles[idx]) - base
+    idx = int(h[3]) + (q - 6 * m) * N.NODE_WORDS + 2
+    base = float(b.records[idx:idx + 1].view(np.float32)[0])
+    b.records[idx] = np.float32(base + delta * max(abs(base), 1e-2)).view(np.uint32)
+    return b, float(b.records[idx:idx + 1].view(np.float32)[0]) - base
+
+
+@pytest.mark.parametrize("c", [0, 17])
+def test_dual_oracle_matches_fp64_differences(c):
+    """Forward-mode derivatives of the FP64 oracle equal its central
+    differences (h = 1e-4 relative, float32-exact steps) on 0.1 s
+    flat-ground trials, where the trajectory is smooth at that scale."""
+    batch = E.synth(E.synth_spec(3, 64)).subset([c])
+    p, run = E.run_for(batch, E.world(duration=0.1, substeps=8))
+    _, g = oracle.full_gradient(batch, p, run, 0, "f64")
+    fd = np.zeros_like(g)
+    for q in range(g.size):
+        up, du = _perturbed(batch, 0, q, 1e-4)
+        dn, dd = _perturbed(batch, 0, q, -1e-4)
+        fd[q] = (_fitness64(up, p, run, 0) - _fitness64(dn, p, run, 0)) / (du - dd)
+    assert np.abs(g).max() > 0
+    assert np.abs(g - fd).max() / np.abs(fd).max() < 1e-5
+
+
+def test_fp32_dual_follows_the_fp32_trajectory():
+    """The FP32 dual build's value part is the FP32 oracle's fitness."""
+    batch = E.synth(E.synth_spec(3, 16))
+    p, run = E.run_for(batch, E.world(duration=1.0, substeps=8))
+    res, _, _ = oracle.run_f32(batch, p, run)
+    for c in range(16):
+        n = oracle.param_count(batch, c)
+        fit, _, _ = oracle.gradient(batch, p, run, c, np.zeros((1, n)), "f32")
+        assert np.float32(fit) == res["fitness"][c]
+
+
+def test_gene_gradient_matches_gene_differences():
+    """The chain rule from record parameters to genes (period, phase,
+    contracted and extended length, strength, friction) against central
+    differences through the record builder, on a smooth 0.1 s trial."""
+    batch = E.synth(E.synth_spec(3, 8)).subset([3])
+    genes = A.genes_of(batch, 0)
+    base = A.records_from_genes(batch, 0, genes)
+    p, run = E.run_for(base, E.world(duration=0.1, substeps=8))
+    _, g = oracle.full_gradient(base, p, run, 0, "f64")
+    gg = A.gene_gradient(base, 0, g)
+    worst = 0.0
+    for key in ("period", "phase", "contracted", "extended", "strength", "friction"):
+        for k in range(len(genes[key])):
+            h = 1e-3 * max(abs(genes[key][k]), 1e-2)
+            vals = []
+            for sgn in (1, -1):
+                moved = {kk: v.copy() for kk, v in genes.items()}
+                moved[key][k] += sgn * h
+                vals.append(_fitness64(A.records_from_genes(base, 0, moved), p, run, 0))
+            fd = (vals[0] - vals[1]) / (2 * h)
+            scale = max(np.abs(gg[key]).max(), 1e-12)
+            worst = max(worst, abs(gg[key][k] - fd) / scale)
+    assert worst < 2e-3, worst
+
+
+def test_ascend_moves_genes_within_their_ranges():
+    batch = E.synth(E.synth_spec(3, 4))
+    grads = np.random.default_rng(1).normal(size=(4, A.STRIDE))
+    moved = A.ascend(batch, range(4), grads, rate=0.05)
+    for c in range(4):
+        a, b = A.genes_of(batch, c), A.genes_of(moved, c)
+        assert not np.allclose(a["period"], b["period"])
+        assert np.all((b["period"] >= 0.5 - 1e-6) & (b["period"] <= 2.0 + 1e-6))
+        assert np.all((b["strength"] >= 0.1 - 1e-6) & (b["strength"] <= 1.0 + 1e-6))
+    moved.validate()
+
+
+def test_adjoint_refuses_what_it_cannot_differentiate():
+    import ctypes as C
+
+    batch = E.synth(E.synth_spec(3, 4))
+    p, run = E.run_for(batch, E.world(duration=1.0, substeps=8, mud=0.05))
+    lib = N.lib()
+    dummy = C.c_void_p(16)
+    assert lib.evo_adjoint_device(dummy, dummy, dummy, dummy, C.byref(p), C.byref(run), 60, dummy, dummy, dummy,
+                                  1 << 40, None) == N.EVO_ERR_INVALID
+    p, run = E.run_for(batch, E.world(duration=1.0, substeps=8))
+    run.max_nodes = 13
+    assert lib.evo_adjoint_device(dummy, dummy, dummy, dummy, C.byref(p), C.byref(run), 60, dummy, dummy, dummy,
+                                  1 << 40, None) == N.EVO_ERR_INVALID
+    run.max_nodes = 10
+    assert lib.evo_adjoint_device(dummy, dummy, dummy, dummy, C.byref(p), C.byref(run), 60, dummy, dummy, dummy,
+                                  16, None) == N.EVO_ERR_INVALID  # workspace too small
+    need = lib.evo_adjoint_workspace_bytes(4, p.steps, 8, 60)
+    assert need == (1 + 60 * 8) * (4 * 12 + 24) * 4 * 4
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("terrain", [0.0, 0.08])
+def test_gpu_adjoint_matches_the_dual_oracle(gpu, terrain):
+    """Flat and 8 cm rough ground, 1 s trials, checkpoints every 20 steps:
+    the recomputed fitness equals the forward launch's bit for bit, and every
+    creature's gradient agrees with the FP32 dual oracle's to 1e-3 of its
+    largest component."""
+    import torch
+
+    batch = E.synth(E.synth_spec(3, 96))
+    p, run = E.run_for(batch, E.world(duration=1.0, substeps=8, terrain=terrain))
+    pair = A.Pair(E.DeviceBatch(batch), p, run, seg=20)
+    pair.forward()
+    pair.backward()
+    torch.cuda.synchronize()
+    res = pair.db.results_numpy()
+    assert np.array_equal(pair.fitness.cpu().numpy().view(np.uint32), res["fitness"].view(np.uint32))
+    g = pair.gradient_numpy()
+    for c in range(0, 96, 8):
+        _, want = oracle.full_gradient(batch, p, run, c, "f32")
+        n = want.size
+        assert np.all(g[c, n:] == 0.0)
+        err = np.abs(g[c, :n] - want).max() / np.abs(want).max()
+        assert err < 1e-3, (c, err)
+
+
+@pytest.mark.gpu
+def test_gpu_adjoint_segments_do_not_change_the_gradient(gpu):
+    """Checkpoint spacing changes only where the trajectory is recomputed
+    from: the gradients are identical for 7, 20 and 60 steps per segment."""
+    import torch
+
+    batch = E.synth(E.synth_spec(3, 64))
+    p, run = E.run_for(batch, E.world(duration=1.0, substeps=8))
+    grads = []
+    for seg in (7, 20, 60):
+        pair = A.Pair(E.DeviceBatch(batch), p, run, seg=seg)
+        pair.forward()
+        pair.backward()
+        torch.cuda.synchronize()
+        grads.append(pair.gradient_numpy())
+    assert np.array_equal(grads[0], grads[1]) and np.array_equal(grads[1], grads[2])
