@@ -101,6 +101,23 @@ class WildMix(C.Structure):
     _fields_ = [("count", C.c_uint32), ("effect", C.c_uint32 * 3), ("level", C.c_uint32 * 3)]
 
 
+class EngineConfig(C.Structure):
+    _fields_ = [("device", C.c_int32), ("standard_slots", C.c_uint32), ("big_unit", C.c_uint32),
+                ("big_slots", C.c_uint32), ("device_budget", C.c_uint64)]
+
+
+class Pack(C.Structure):
+    _fields_ = [("records", C.c_void_p), ("record_words", C.c_uint64), ("muscles", C.c_void_p),
+                ("muscle_floats", C.c_uint64), ("heads", C.c_void_p), ("count", C.c_uint32), ("slots", C.c_void_p)]
+
+
+class Completion(C.Structure):
+    _fields_ = [("ticket", C.c_uint64), ("count", C.c_uint32), ("recorded", C.c_uint32),
+                ("frame_floats", C.c_uint64), ("gpu_seconds", C.c_double)]
+
+
+SLOT_STANDARD, SLOT_CONFIRM = 0, 1
+
 EFFECTS = 16
 WILD_ISLANDS = 100
 
@@ -150,6 +167,20 @@ SIGNATURES = {
     "evo_splitmix_finalize": ([_u64], _u64),
     "evo_adjoint_workspace_bytes": ([_u32, _u32, _u32, _u32], _u64),
     "evo_adjoint_device": ([_P, _P, _P, _P, C.POINTER(Params), C.POINTER(Run), _u32, _P, _P, _P, _u64, _P], C.c_int),
+    "evo_engine_default": ([C.POINTER(EngineConfig)], None),
+    "evo_open": ([C.POINTER(EngineConfig), C.POINTER(C.c_void_p)], C.c_int),
+    "evo_close": ([_P], None),
+    "evo_free_slots": ([_P], _u32),
+    "evo_free_slots_for": ([_P, _u32], _u32),
+    "evo_confirm_free": ([_P], C.c_int),
+    "evo_replay_free": ([_P], C.c_int),
+    "evo_submit": ([_P, C.POINTER(Pack), C.POINTER(WorldConfig), _u32, _P, C.POINTER(_u64)], C.c_int),
+    "evo_record_frame_floats": ([C.POINTER(Pack), C.POINTER(WorldConfig)], _u64),
+    "evo_record": ([_P, C.POINTER(Pack), C.POINTER(WorldConfig), _P, _P, _u64, C.POINTER(_u64)], C.c_int),
+    "evo_poll": ([_P, _u32, C.POINTER(Completion)], C.c_int),
+    "evo_release_idle": ([_P], _u64),
+    "evo_allocated_bytes": ([_P], _u64),
+    "evo_last_error": ([_P], C.c_char_p),
     "evo_validate_batch": ([_P, _u64, _P, _u64, _P, _u32, C.POINTER(_u32)], C.c_int),
     "evo_max_muscles": ([_P, _u32, _u32], _u32),
     "evo_build_info": ([], C.c_char_p),

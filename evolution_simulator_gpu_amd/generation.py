@@ -157,8 +157,15 @@ class PerShapeGeneration:
             self.launches.append((p, run))
         self.streams = [torch.cuda.Stream() for _ in range(streams)]
 
-    def step(self) -> None:
+    def step(self, progress=None) -> None:
+        """Queues every launch; with `progress`, waits for them, reporting
+        every 20 s how many have finished (without serializing them)."""
+        import time as _time
+
+        import torch
+
         lib = N.lib()
+        marks = []
         for k, (p, run) in enumerate(self.launches):
             i = k % len(self.streams)
             rc = lib.evo_advance_device(self.db.records.data_ptr(), self.db.muscles.data_ptr(),
@@ -167,6 +174,20 @@ class PerShapeGeneration:
                                         C.c_void_p(self.streams[i].cuda_stream))
             if rc != N.EVO_OK:
                 raise RuntimeError(f"evo_advance_device failed: {rc}")
+            if progress is not None:
+                ev = torch.cuda.Event()
+                ev.record(self.streams[i])
+                marks.append(ev)
+        if progress is not None:
+            last = _time.perf_counter()
+            while True:
+                done = sum(1 for ev in marks if ev.query())
+                if done == len(marks):
+                    break
+                if _time.perf_counter() - last > 20.0:
+                    progress(f"per-shape launches finished: {done} of {len(marks)}")
+                    last = _time.perf_counter()
+                _time.sleep(0.05)
 
     def results_in_packed_order(self) -> np.ndarray:
         """Results mapped back to the order of the batch it was built from."""

@@ -276,6 +276,91 @@ int evo_describe_device(const uint32_t* d_records, const uint32_t* d_heads, cons
                         uint32_t base, uint32_t count, uint32_t steps, uint32_t rate, uint32_t layout,
                         evo_metrics* d_out, void* stream);
 
+/* ---------------------------------------------------------------- engine */
+
+/* A submission engine on one GPU: the device side of the reference's
+ * CudaEngine behind its `Device` contract (src/engine.rs:430-467,
+ * src/cuda_engine.rs:1475-2392). Standard units run on `standard_slots`
+ * slots (a unit of more than `big_unit` creatures only on the first
+ * `big_slots`), confirmation trials on a slot of their own or a free
+ * standard one, recordings on a replay slot. Each slot keeps its device and
+ * pinned buffers for reuse. One thread uses a handle at a time. */
+typedef struct evo_engine evo_engine;
+
+#define EVO_SLOT_STANDARD 0u
+#define EVO_SLOT_CONFIRM 1u
+
+typedef struct evo_engine_config {
+    int32_t device;           /* device ordinal */
+    uint32_t standard_slots;  /* 24 in the reference (src/engine.rs:390-396); at most 64 */
+    uint32_t big_unit;        /* 8192 (src/cuda_engine.rs:505) */
+    uint32_t big_slots;       /* 4 (src/cuda_engine.rs:510) */
+    uint64_t device_budget;   /* bytes of buffers the engine may hold; 0 = no limit */
+} evo_engine_config;
+
+/* A packed unit in host memory: the buffers of evo_validate_batch and the
+ * caller's index of each packed creature (`slots`, a permutation; NULL for
+ * the identity), src/creature_kernel.rs:280-310. */
+typedef struct evo_pack {
+    const uint32_t* records;
+    uint64_t record_words;
+    const float* muscles;
+    uint64_t muscle_floats;
+    const uint32_t* heads;
+    uint32_t count;
+    const uint32_t* slots;
+} evo_pack;
+
+/* A finished submission: its ticket, creatures, whether it recorded frames
+ * (and how many floats), and the GPU time of its trials. */
+typedef struct evo_completion {
+    uint64_t ticket;
+    uint32_t count;
+    uint32_t recorded;
+    uint64_t frame_floats;
+    double gpu_seconds;
+} evo_completion;
+
+void evo_engine_default(evo_engine_config* c);
+/* EVO_ERR_DEVICE when the device cannot be opened. */
+int evo_open(const evo_engine_config* cfg, evo_engine** out);
+/* Waits for submissions in flight and frees everything. */
+void evo_close(evo_engine* e);
+uint32_t evo_free_slots(const evo_engine* e);
+uint32_t evo_free_slots_for(const evo_engine* e, uint32_t creatures);
+int evo_confirm_free(const evo_engine* e);
+int evo_replay_free(const evo_engine* e);
+
+/* Queues the whole trials of a unit in `world`. The pack is copied before the
+ * call returns; results_out (count entries) must stay valid until the poll
+ * that reports the ticket, which writes the results there in the caller's
+ * order. EVO_ERR_BUSY without a free slot, EVO_ERR_OOM when buffers cannot
+ * grow (nothing queued: retry later with a smaller unit or after polls),
+ * EVO_ERR_INVALID for a malformed pack, EVO_ERR_DEVICE after a device error
+ * (the engine must be reopened). */
+int evo_submit(evo_engine* e, const evo_pack* pack, const evo_world_config* world, uint32_t kind,
+               evo_result* results_out, uint64_t* ticket);
+
+/* Floats a recording of this pack writes: every frame of every creature
+ * (settle + steps + 1 frames of 2 * 32 + max muscles + 1 float pairs). */
+uint64_t evo_record_frame_floats(const evo_pack* pack, const evo_world_config* world);
+
+/* Queues a recorded trial on the replay slot (src/cuda_engine.rs:2046-2057);
+ * frames_out receives each creature's frames at its caller index. */
+int evo_record(evo_engine* e, const evo_pack* pack, const evo_world_config* world, evo_result* results_out,
+               float* frames_out, uint64_t frame_floats, uint64_t* ticket);
+
+/* The finished submission with the lowest ticket, waiting up to timeout_us
+ * for one: EVO_OK with *out filled, or EVO_ERR_NOT_READY. */
+int evo_poll(evo_engine* e, uint32_t timeout_us, evo_completion* out);
+
+/* Frees the buffers of idle slots; returns the bytes freed. */
+uint64_t evo_release_idle(evo_engine* e);
+/* Device and pinned host bytes the engine holds. */
+uint64_t evo_allocated_bytes(const evo_engine* e);
+/* What the last failed call reported. */
+const char* evo_last_error(const evo_engine* e);
+
 /* Host check of a packed batch: counts within the simulator's limits
  * (2-32 nodes, 0-96 muscles), every node index in range, bone pivots before
  * their child, parents before their bone, offsets inside the buffers.

@@ -1,0 +1,128 @@
+"""The submission engine (src/engine.rs:430-467 `Device`,
+src/cuda_engine.rs:1475-2392): slots, tickets, caller-order results,
+recordings, memory accounting and the out-of-memory path.
+
+Re-expresses the reference's engine tests (src/engine.rs:1187-1767 run the
+same contract against a fake device): a full engine refuses new units, a
+finished unit's results arrive in unit order, a replay's result is the
+trial's result, idle buffers are released.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import evolution_simulator_gpu_amd as E
+from evolution_simulator_gpu_amd import _native as N
+from evolution_simulator_gpu_amd.engine import Engine, EngineError
+from tests.conftest import gpu_available
+from tests.helpers import bits
+
+
+def test_open_without_a_device_is_a_device_error():
+    if gpu_available():
+        pytest.skip("a GPU is present")
+    cfg = N.EngineConfig()
+    N.lib().evo_engine_default(C.byref(cfg))
+    h = C.c_void_p(None)
+    assert N.lib().evo_open(C.byref(cfg), C.byref(h)) == N.EVO_ERR_DEVICE
+    assert not h.value
+    cfg.standard_slots = 0
+    assert N.lib().evo_open(C.byref(cfg), C.byref(h)) == N.EVO_ERR_INVALID
+
+
+def test_engine_defaults_follow_the_reference():
+    cfg = N.EngineConfig()
+    N.lib().evo_engine_default(C.byref(cfg))
+    assert (cfg.standard_slots, cfg.big_unit, cfg.big_slots, cfg.device_budget) == (24, 8192, 4, 0)
+
+
+def _caller_order(batch, packed_results):
+    out = np.empty_like(packed_results)
+    out[batch.slots] = packed_results
+    return out
+
+
+@pytest.mark.gpu
+def test_units_complete_in_caller_order(gpu):
+    eng = Engine(standard_slots=2, big_unit=1000, big_slots=1)
+    units = [(E.synth(E.synth_spec(1, 1500, seed=7)), E.world(duration=2.0, substeps=8)),
+             (E.synth(E.synth_spec(0, 700, seed=8)), E.world(duration=1.0, substeps=16, terrain=0.08)),
+             (E.synth(E.synth_spec(2, 300, seed=9)), E.world(duration=1.0, substeps=8, gravity=19.6))]
+    assert eng.free_slots() == 2 and eng.free_slots(1500) == 1
+    t0 = eng.submit(*units[0])
+    assert eng.free_slots(1500) == 0 and eng.free_slots(700) == 1
+    t1 = eng.submit(*units[1])
+    with pytest.raises(EngineError) as busy:
+        eng.submit(*units[2])
+    assert busy.value.code == N.EVO_ERR_BUSY
+    assert eng.confirm_free()
+    t2 = eng.submit(*units[2], confirm=True)
+    got = {}
+    while len(got) < 3:
+        r = eng.poll(2_000_000)
+        assert r is not None
+        ticket, res, frames, seconds = r
+        assert frames is None and seconds > 0
+        got[ticket] = res
+    for t, (batch, w) in zip((t0, t1, t2), units):
+        want = _caller_order(batch, E.simulate(batch, w))
+        assert np.array_equal(bits(got[t]), bits(want))
+    assert eng.poll(0) is None
+    held = eng.allocated_bytes()
+    assert held > 0 and eng.release_idle() == held and eng.allocated_bytes() == 0
+    eng.close()
+
+
+@pytest.mark.gpu
+def test_recording_on_the_replay_slot(gpu):
+    eng = Engine(standard_slots=1)
+    batch = E.synth(E.synth_spec(1, 3, seed=3))
+    w = E.world(duration=1.0, substeps=8)
+    assert eng.replay_free()
+    t = eng.record(batch, w)
+    assert not eng.replay_free()
+    ticket, res, frames, _ = eng.poll(5_000_000)
+    assert ticket == t and eng.replay_free()
+    want, want_frames = E.simulate(batch, w, record=True)
+    assert np.array_equal(bits(res), bits(_caller_order(batch, want)))
+    per = want_frames.size // batch.count
+    for c in range(batch.count):
+        a = frames[batch.slots[c] * per:(batch.slots[c] + 1) * per]
+        b = want_frames[c * per:(c + 1) * per]
+        nan = np.isnan(a) & np.isnan(b)
+        assert np.array_equal(a.view(np.uint32)[~nan], b.view(np.uint32)[~nan])
+    eng.close()
+
+
+@pytest.mark.gpu
+def test_out_of_memory_leaves_the_engine_usable(gpu):
+    eng = Engine(standard_slots=2, device_budget=12 << 20)
+    big = E.synth(E.synth_spec(1, 60000, seed=1))
+    with pytest.raises(EngineError) as oom:
+        eng.submit(big, E.world(duration=1.0, substeps=8))
+    assert oom.value.out_of_memory
+    eng.release_idle()
+    small = E.synth(E.synth_spec(1, 200, seed=2))
+    w = E.world(duration=1.0, substeps=8)
+    eng.submit(small, w)
+    _, res, _, _ = eng.poll(5_000_000)
+    assert np.array_equal(bits(res), bits(_caller_order(small, E.simulate(small, w))))
+    eng.close()
+
+
+@pytest.mark.gpu
+def test_malformed_units_are_refused(gpu):
+    eng = Engine(standard_slots=1)
+    batch = E.synth(E.synth_spec(1, 50))
+    bad = E.Batch(batch.records.copy(), batch.muscles.copy(), batch.heads.copy(), batch.slots.copy())
+    bad.heads[3, 0] = (int(bad.heads[3, 0]) & 0xFFFFFF00) | 40
+    with pytest.raises(EngineError) as err:
+        eng.submit(bad, E.world())
+    assert err.value.code == N.EVO_ERR_INVALID
+    dup = E.Batch(batch.records, batch.muscles, batch.heads, batch.slots.copy())
+    dup.slots[1] = dup.slots[0]
+    with pytest.raises(EngineError):
+        eng.submit(dup, E.world())
+    assert eng.free_slots() == 1
+    eng.close()
