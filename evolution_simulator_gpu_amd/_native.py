@@ -118,6 +118,27 @@ class Completion(C.Structure):
 
 SLOT_STANDARD, SLOT_CONFIRM = 0, 1
 
+
+class NodeGene(C.Structure):
+    _fields_ = [("x", C.c_float), ("y", C.c_float), ("diameter", C.c_float), ("friction", C.c_float)]
+
+
+class BoneGene(C.Structure):
+    _fields_ = [("a", C.c_uint32), ("b", C.c_uint32), ("rest_length", C.c_float), ("min_angle", C.c_float),
+                ("max_angle", C.c_float), ("organ_mass", C.c_float), ("organ_at", C.c_float)]
+
+
+class MuscleGene(C.Structure):
+    _fields_ = [("bone_a", C.c_uint32), ("bone_b", C.c_uint32), ("anchor_a", C.c_float), ("anchor_b", C.c_float),
+                ("shortest", C.c_float), ("longest", C.c_float), ("period", C.c_float), ("phase", C.c_float),
+                ("duty", C.c_float), ("stiffness", C.c_float), ("sensor", C.c_uint32), ("reset", C.c_float),
+                ("tendon", C.c_float)]
+
+
+class Genome(C.Structure):
+    _fields_ = [("nodes", C.c_void_p), ("node_count", C.c_uint32), ("bones", C.c_void_p), ("bone_count", C.c_uint32),
+                ("muscles", C.c_void_p), ("muscle_count", C.c_uint32), ("id", C.c_uint64), ("flags", C.c_uint32)]
+
 EFFECTS = 16
 WILD_ISLANDS = 100
 
@@ -181,6 +202,8 @@ SIGNATURES = {
     "evo_release_idle": ([_P], _u64),
     "evo_allocated_bytes": ([_P], _u64),
     "evo_last_error": ([_P], C.c_char_p),
+    "evo_pack_genome_sizes": ([_P, _u32, C.POINTER(_u64), C.POINTER(_u64)], C.c_int),
+    "evo_pack_genomes": ([_P, _u32, C.POINTER(WorldConfig), _P, _P, _P, _P, C.c_int], C.c_int),
     "evo_validate_batch": ([_P, _u64, _P, _u64, _P, _u32, C.POINTER(_u32)], C.c_int),
     "evo_max_muscles": ([_P, _u32, _u32], _u32),
     "evo_build_info": ([], C.c_char_p),

@@ -1,0 +1,442 @@
+// Genomes -> packed batches: the host side of a generation.
+//
+// Restates, in the reference's f32 operation order:
+//   physics::node_mass, body, add_bone_masses, add_muscle_masses,
+//   slack_length                       src/physics.rs:182-296
+//   physics2::Model::new               src/physics2.rs:190-342
+//   Model::start, kinematics, center   src/physics2.rs:365-427
+//   kernel::fill_creature              src/kernel.rs:336-411
+//   kernel::pack_reusing               src/kernel.rs:471-593
+// The trigonometry is the C library's sinf, cosf and atan2f, the functions
+// Rust's f32 methods call on this platform. Creatures are filled in
+// parallel, each into its own ranges of the output.
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdint>
+#include <numeric>
+#include <thread>
+#include <vector>
+
+#include "host_common.hpp"
+
+namespace {
+
+constexpr float kTau = 6.28318530717958647692f;
+constexpr float kPi = 3.14159265358979323846f;
+constexpr float kMuscleSpeed = 24.0f, kMuscleForce = 200.0f, kMuscleEnergy = 120.0f, kBoneDensity = 4.0f;
+constexpr float kHillSpeed = 8.0f, kDrivenAcceleration = 200.0f, kTendonStretch = 0.25f;
+constexpr float kMuscleMassBase = 0.05f, kMuscleMassPerM = 1.0f;
+constexpr uint32_t kNoSensor = 255u;
+
+struct Node {
+    float x, y, radius, friction, mass;
+};
+
+// src/physics.rs:182-184
+float node_mass(float diameter) {
+    const float r = diameter / 0.08f;
+    return std::min(std::max(0.1f * (r * r), 0.02f), 10.0f);
+}
+
+bool bone_point(const evo_genome& g, uint32_t bone, float t, const std::vector<Node>& nodes, float* p) {
+    if (bone >= g.bone_count) { return false; }
+    const evo_bone_gene& b = g.bones[bone];
+    if (b.a >= nodes.size() || b.b >= nodes.size()) { return false; }
+    const Node &a = nodes[b.a], &c = nodes[b.b];
+    p[0] = a.x + (c.x - a.x) * t;
+    p[1] = a.y + (c.y - a.y) * t;
+    return true;
+}
+
+// src/physics.rs:266-279
+float slack_length(const evo_genome& g, const std::vector<Node>& nodes, const evo_muscle_gene& m) {
+    float p[2], q[2];
+    float span = 0.0f;
+    if (bone_point(g, m.bone_a, m.anchor_a, nodes, p) && bone_point(g, m.bone_b, m.anchor_b, nodes, q)) {
+        const float dx = p[0] - q[0], dy = p[1] - q[1];
+        span = std::sqrt(dx * dx + dy * dy);
+    }
+    return std::max(m.longest, span);
+}
+
+// src/physics.rs:186-296: node masses with bones, organs and muscles.
+std::vector<Node> body_nodes(const evo_genome& g) {
+    std::vector<Node> nodes(g.node_count);
+    for (uint32_t i = 0; i < g.node_count; i++) {
+        const evo_node_gene& n = g.nodes[i];
+        nodes[i] = Node{n.x, n.y, n.diameter * 0.5f, n.friction, node_mass(n.diameter)};
+    }
+    for (uint32_t j = 0; j < g.bone_count; j++) {
+        const evo_bone_gene& b = g.bones[j];
+        if (b.a >= g.node_count || b.b >= g.node_count) { continue; }
+        const float half = 0.5f * kBoneDensity * b.rest_length * b.rest_length;
+        nodes[b.a].mass += half;
+        nodes[b.b].mass += half;
+        if (b.organ_mass > 0.0f) {
+            nodes[b.a].mass += b.organ_mass * (1.0f - b.organ_at);
+            nodes[b.b].mass += b.organ_mass * b.organ_at;
+        }
+    }
+    for (uint32_t k = 0; k < g.muscle_count; k++) {
+        const evo_muscle_gene& m = g.muscles[k];
+        const float half = 0.5f * (kMuscleMassBase + kMuscleMassPerM * slack_length(g, nodes, m));
+        const uint32_t bones[2] = {m.bone_a, m.bone_b};
+        const float ts[2] = {m.anchor_a, m.anchor_b};
+        for (int s = 0; s < 2; s++) {
+            if (bones[s] >= g.bone_count) { continue; }
+            const evo_bone_gene& b = g.bones[bones[s]];
+            if (b.a >= g.node_count || b.b >= g.node_count) { continue; }
+            const float t = std::min(std::max(ts[s], 0.0f), 1.0f);
+            nodes[b.a].mass += half * (1.0f - t);
+            nodes[b.b].mass += half * t;
+        }
+    }
+    return nodes;
+}
+
+float wrap(float a) { return a - kTau * std::floor((a + kPi) / kTau); }
+
+struct MuscleModel {
+    uint32_t bone_a, bone_b;
+    float anchor_a, anchor_b, hill, slack, tendon_k, amplitude, inv_period, phase, duty, inv_duty, inv_complement;
+    float stiffness, strength, reset;
+    uint32_t sensor;  // renumbered node, or EVO_NONE
+};
+
+// src/physics2.rs:190-342, in renumbered form.
+struct Model {
+    uint32_t n = 0, b = 0;
+    std::vector<float> mass, radius, friction, length, lo, hi, rest;
+    std::vector<uint32_t> pivot;
+    std::vector<int32_t> parent;  // -1 for none
+    std::vector<MuscleModel> muscles;
+    std::vector<float> sx, sy;  // gene positions in renumbered order
+    float total_mass = 0.0f, inv_mass = 0.0f, amplitude = 0.0f, quake_phase = 0.0f;
+};
+
+Model build(const evo_genome& g, const evo_world_config& cfg) {
+    Model md;
+    const std::vector<Node> nodes = body_nodes(g);
+    const uint32_t n = g.node_count, nb = g.bone_count;
+    md.n = n;
+    md.b = nb;
+    std::vector<uint32_t> order(n);
+    order[0] = 0;
+    for (uint32_t j = 0; j < nb; j++) { order[j + 1] = g.bones[j].b; }
+    std::vector<uint32_t> record(n, 0);
+    for (uint32_t r = 0; r < n; r++) { record[order[r]] = r; }
+    std::vector<int32_t> parent_of_node(n, -1);
+    for (uint32_t j = 0; j < nb; j++) { parent_of_node[g.bones[j].b] = static_cast<int32_t>(j); }
+    auto angle = [&](uint32_t j) {
+        const evo_node_gene &p = g.nodes[g.bones[j].a], &q = g.nodes[g.bones[j].b];
+        return std::atan2(q.y - p.y, q.x - p.x);
+    };
+    for (uint32_t j = 0; j < nb; j++) {
+        const evo_bone_gene& bone = g.bones[j];
+        int32_t up = parent_of_node[bone.a];
+        if (up < 0 && j > 0) { up = 0; }
+        md.parent.push_back(up);
+        const float relative = up >= 0 ? wrap(angle(j) - angle(static_cast<uint32_t>(up))) : angle(j);
+        md.rest.push_back(relative);
+        md.lo.push_back(relative + bone.min_angle);
+        md.hi.push_back(relative + bone.max_angle);
+    }
+    for (uint32_t k = 0; k < g.muscle_count; k++) {
+        const evo_muscle_gene& m = g.muscles[k];
+        const evo_bone_gene &ba = g.bones[m.bone_a], &bb = g.bones[m.bone_b];
+        const uint32_t ends[4] = {ba.a, ba.b, bb.a, bb.b};
+        MuscleModel mm{};
+        mm.bone_a = m.bone_a;
+        mm.bone_b = m.bone_b;
+        mm.anchor_a = m.anchor_a;
+        mm.anchor_b = m.anchor_b;
+        mm.hill = 1.0f / (kHillSpeed * std::max(m.longest, 0.05f));
+        mm.slack = slack_length(g, nodes, m);
+        mm.tendon_k = 0.0f;
+        mm.amplitude = std::min(m.longest - m.shortest,
+                                2.0f * kMuscleSpeed * m.period * std::min(m.duty, 1.0f - m.duty) / kPi);
+        mm.inv_period = 1.0f / m.period;
+        mm.phase = m.phase;
+        mm.duty = m.duty;
+        mm.inv_duty = 1.0f / m.duty;
+        mm.inv_complement = 1.0f / (1.0f - m.duty);
+        mm.stiffness = m.stiffness;
+        mm.strength = 1.0f;
+        mm.sensor = m.sensor != kNoSensor ? record[ends[m.sensor]] : EVO_NONE;
+        mm.reset = m.reset;
+        md.muscles.push_back(mm);
+    }
+    // the mass each bone carries, the neck with the head
+    std::vector<float> subtree(nb);
+    for (uint32_t j = 0; j < nb; j++) { subtree[j] = nodes[order[j + 1]].mass + (j == 0 ? nodes[order[0]].mass : 0.0f); }
+    for (uint32_t j = nb; j-- > 1u;) {
+        if (md.parent[j] >= 0) { subtree[md.parent[j]] += subtree[j]; }
+    }
+    // muscles on the same two bones and side of the joint share a strength
+    struct Key {
+        uint32_t lo, hi;
+        bool side;
+    };
+    auto group = [&](const evo_muscle_gene& m) {
+        const uint32_t a = m.bone_a, b = m.bone_b;
+        const evo_bone_gene &ba = g.bones[a], &bb = g.bones[b];
+        const evo_node_gene &a0 = g.nodes[ba.a], &a1 = g.nodes[ba.b], &b0 = g.nodes[bb.a], &b1 = g.nodes[bb.b];
+        const float pa[2] = {a0.x + (a1.x - a0.x) * m.anchor_a, a0.y + (a1.y - a0.y) * m.anchor_a};
+        const float pb[2] = {b0.x + (b1.x - b0.x) * m.anchor_b, b0.y + (b1.y - b0.y) * m.anchor_b};
+        int32_t shared = -1;
+        for (const uint32_t node : {ba.a, ba.b}) {
+            if (shared < 0 && (node == bb.a || node == bb.b)) { shared = static_cast<int32_t>(node); }
+        }
+        float jx, jy;
+        if (shared >= 0) {
+            jx = g.nodes[shared].x;
+            jy = g.nodes[shared].y;
+        } else {
+            jx = 0.5f * (pa[0] + pb[0]);
+            jy = 0.5f * (pa[1] + pb[1]) - 1.0f;
+        }
+        const float cross = (pa[0] - jx) * (pb[1] - jy) - (pa[1] - jy) * (pb[0] - jx);
+        return Key{std::min(a, b), std::max(a, b), cross >= 0.0f};
+    };
+    std::vector<Key> groups;
+    for (uint32_t k = 0; k < g.muscle_count; k++) { groups.push_back(group(g.muscles[k])); }
+    for (uint32_t k = 0; k < g.muscle_count; k++) {
+        MuscleModel& mm = md.muscles[k];
+        const float driven = std::min(subtree[mm.bone_a], subtree[mm.bone_b]);
+        const Key key = groups[k];
+        uint32_t sharing = 0;
+        for (const Key& o : groups) { sharing += (o.lo == key.lo && o.hi == key.hi && o.side == key.side) ? 1u : 0u; }
+        mm.strength = std::min(kDrivenAcceleration * driven / kMuscleForce, 1.0f) /
+                      static_cast<float>(std::max(sharing, 1u));
+        mm.tendon_k = g.muscles[k].tendon * kMuscleForce * mm.strength / (kTendonStretch * std::max(mm.slack, 0.05f));
+    }
+    const uint32_t quake = evo::quake_hash(g.id);
+    const bool still = cfg.quake <= 0.0f || !cfg.ground;
+    for (uint32_t r = 0; r < n; r++) {
+        const Node& node = nodes[order[r]];
+        md.mass.push_back(node.mass);
+        md.radius.push_back(node.radius);
+        md.friction.push_back(node.friction);
+        md.sx.push_back(g.nodes[order[r]].x);
+        md.sy.push_back(g.nodes[order[r]].y);
+    }
+    float total = 0.0f;
+    for (uint32_t r = 0; r < n; r++) { total += nodes[order[r]].mass; }
+    md.total_mass = total;
+    md.inv_mass = 1.0f / total;
+    for (uint32_t j = 0; j < nb; j++) {
+        md.pivot.push_back(record[g.bones[j].a]);
+        md.length.push_back(g.bones[j].rest_length);
+    }
+    md.quake_phase = still ? 0.0f : evo::quake_phase(quake);
+    md.amplitude = cfg.terrain + (still ? 0.0f : cfg.quake * evo::quake_scale(quake));
+    return md;
+}
+
+// src/physics2.rs:402-417
+void kinematics(const Model& md, float x0, float y0, const float* q, std::vector<float>& px, std::vector<float>& py) {
+    std::vector<float> th(md.b);
+    px[0] = x0;
+    py[0] = y0;
+    for (uint32_t j = 0; j < md.b; j++) {
+        const float t = md.parent[j] >= 0 ? th[md.parent[j]] + q[j] : q[0];
+        th[j] = t;
+        const float s = std::sin(t), c = std::cos(t);
+        const uint32_t p = md.pivot[j], ch = j + 1u;
+        const float l = md.length[j];
+        px[ch] = px[p] + l * c;
+        py[ch] = py[p] + l * s;
+    }
+}
+
+// src/physics2.rs:371-398: the rest pose, its centre of mass over x = 0 and
+// its lowest surface on the ground.
+void start_pose(const Model& md, const evo_world_config& cfg, std::vector<float>& px, std::vector<float>& py) {
+    px.assign(md.n, 0.0f);
+    py.assign(md.n, 0.0f);
+    float x0 = md.sx[0], y0 = md.sy[0];
+    kinematics(md, x0, y0, md.rest.data(), px, py);
+    float cx = 0.0f, cy = 0.0f;
+    for (uint32_t i = 0; i < md.n; i++) {
+        cx += px[i] * md.mass[i];
+        cy += py[i] * md.mass[i];
+    }
+    cx = cx * md.inv_mass;
+    cy = cy * md.inv_mass;
+    float low = INFINITY;
+    for (uint32_t i = 0; i < md.n; i++) {
+        const float h = evo::ground_height(cfg, px[i] - cx, md.amplitude, md.quake_phase);
+        low = std::min(low, py[i] - md.radius[i] - std::max(h, -1e6f));
+    }
+    if (!cfg.ground) { low = 0.0f; }
+    x0 -= cx;
+    y0 -= low;
+    kinematics(md, x0, y0, md.rest.data(), px, py);
+}
+
+// src/kernel.rs:336-411
+void fill(const Model& md, const evo_world_config& cfg, uint32_t* rec, float* mus) {
+    std::vector<float> px, py;
+    start_pose(md, cfg, px, py);
+    std::vector<uint32_t> children(EVO_MAX_NODES, 0u);
+    for (uint32_t j = 1; j < md.b; j++) {
+        if (md.parent[j] >= 0) { children[md.parent[j]]++; }
+    }
+    for (uint32_t i = 0; i < md.n; i++) {
+        bool foot = false;
+        if (i > 0) {
+            const uint32_t j = i - 1u;
+            const int32_t p = md.parent[j];
+            foot = children[j] == 0u && j > 0u && p > 0 && children[p] == 1u;
+        }
+        uint32_t* r = rec + i * EVO_NODE_WORDS;
+        r[0] = evo::as_uint(md.mass[i]);
+        r[1] = evo::as_uint(md.radius[i]);
+        r[2] = evo::as_uint(md.friction[i]);
+        r[3] = evo::as_uint(px[i]);
+        r[4] = evo::as_uint(py[i]);
+        r[5] = foot ? 1u : 0u;
+    }
+    uint32_t* bones = rec + md.n * EVO_NODE_WORDS;
+    for (uint32_t j = 0; j < md.b; j++) {
+        uint32_t* r = bones + j * EVO_BONE_WORDS;
+        r[0] = md.pivot[j];
+        r[1] = evo::as_uint(md.length[j]);
+        r[2] = (j > 0u && md.parent[j] >= 0) ? static_cast<uint32_t>(md.parent[j]) : EVO_NONE;
+        r[3] = evo::as_uint(md.lo[j]);
+        r[4] = evo::as_uint(md.hi[j]);
+        r[5] = 0u;
+    }
+    for (uint32_t k = 0; k < md.muscles.size(); k++) {
+        const MuscleModel& m = md.muscles[k];
+        const float strength = m.strength * 1.0f;
+        const uint32_t a0 = md.pivot[m.bone_a], b0 = md.pivot[m.bone_b];
+        const uint32_t nodes = a0 | ((m.bone_a + 1u) << 8u) | (b0 << 16u) | ((m.bone_b + 1u) << 24u);
+        float* f = mus + k * EVO_MUSCLE_WORDS;
+        f[0] = evo::as_float(nodes);
+        f[1] = m.anchor_a;
+        f[2] = m.anchor_b;
+        f[3] = m.hill;
+        f[4] = kMuscleForce * strength;
+        f[5] = 1.0f / (kMuscleEnergy * strength);
+        f[6] = m.tendon_k;
+        f[7] = m.slack;
+        f[8] = m.amplitude;
+        f[9] = m.inv_period;
+        f[10] = m.phase;
+        f[11] = m.duty;
+        f[12] = m.inv_duty;
+        f[13] = m.inv_complement;
+        f[14] = m.stiffness;
+        f[15] = m.reset;
+        f[16] = evo::as_float(m.sensor);
+        f[17] = f[18] = f[19] = 0.0f;
+    }
+}
+
+// A genome the simulator can run: a tree in canonical order (bone j joins
+// node a, the head or the child of an earlier bone, to its own child b;
+// bone 0 is the neck at the head), within the size limits.
+bool genome_ok(const evo_genome& g) {
+    if (g.node_count < 2u || g.node_count > EVO_MAX_NODES || g.muscle_count > EVO_MAX_MUSCLES) { return false; }
+    if (g.bone_count + 1u != g.node_count || !g.nodes || !g.bones || (g.muscle_count && !g.muscles)) { return false; }
+    std::vector<char> reached(g.node_count, 0);
+    reached[0] = 1;
+    for (uint32_t j = 0; j < g.bone_count; j++) {
+        const evo_bone_gene& b = g.bones[j];
+        if (b.a >= g.node_count || b.b >= g.node_count || !reached[b.a] || reached[b.b]) { return false; }
+        if (j == 0 && b.a != 0u) { return false; }
+        reached[b.b] = 1;
+        if (!(b.rest_length > 0.0f)) { return false; }
+    }
+    for (uint32_t i = 0; i < g.node_count; i++) {
+        if (!(g.nodes[i].diameter > 0.0f)) { return false; }
+    }
+    for (uint32_t k = 0; k < g.muscle_count; k++) {
+        const evo_muscle_gene& m = g.muscles[k];
+        if (m.bone_a >= g.bone_count || m.bone_b >= g.bone_count) { return false; }
+        if (m.sensor != kNoSensor && m.sensor >= 4u) { return false; }
+        if (!(m.period > 0.0f) || !(m.duty > 0.0f && m.duty < 1.0f)) { return false; }
+    }
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int evo_pack_genome_sizes(const evo_genome* genomes, uint32_t count, uint64_t* record_words, uint64_t* muscle_floats) {
+    if (!genomes || !record_words || !muscle_floats) { return EVO_ERR_INVALID; }
+    uint64_t rw = 0, mf = 0;
+    for (uint32_t i = 0; i < count; i++) {
+        if (!genome_ok(genomes[i])) { return EVO_ERR_INVALID; }
+        rw += static_cast<uint64_t>(genomes[i].node_count) * EVO_NODE_WORDS +
+              static_cast<uint64_t>(genomes[i].bone_count) * EVO_BONE_WORDS;
+        mf += static_cast<uint64_t>(genomes[i].muscle_count) * EVO_MUSCLE_WORDS;
+    }
+    *record_words = rw;
+    *muscle_floats = mf;
+    return EVO_OK;
+}
+
+int evo_pack_genomes(const evo_genome* genomes, uint32_t count, const evo_world_config* world, uint32_t* records,
+             float* muscles, uint32_t* heads, uint32_t* slots, int threads) {
+    if (!genomes || !world || !records || !muscles || !heads) { return EVO_ERR_INVALID; }
+    for (uint32_t i = 0; i < count; i++) {
+        if (!genome_ok(genomes[i])) { return EVO_ERR_INVALID; }
+    }
+    // src/kernel.rs:487-493: muscles, then nodes, then index
+    std::vector<uint32_t> order(count);
+    std::iota(order.begin(), order.end(), 0u);
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+        if (genomes[a].muscle_count != genomes[b].muscle_count) {
+            return genomes[a].muscle_count < genomes[b].muscle_count;
+        }
+        return genomes[a].node_count < genomes[b].node_count;
+    });
+    std::vector<uint64_t> rec_at(count), mus_at(count);
+    uint64_t rw = 0, mf = 0;
+    for (uint32_t c = 0; c < count; c++) {
+        const evo_genome& g = genomes[order[c]];
+        rec_at[c] = rw;
+        mus_at[c] = mf;
+        rw += static_cast<uint64_t>(g.node_count) * EVO_NODE_WORDS + static_cast<uint64_t>(g.bone_count) * EVO_BONE_WORDS;
+        mf += static_cast<uint64_t>(g.muscle_count) * EVO_MUSCLE_WORDS;
+    }
+    if (rw > 0xffffffffull || mf > 0xffffffffull) { return EVO_ERR_INVALID; }  // head offsets are 32-bit words
+    std::atomic<uint32_t> next{0};
+    auto work = [&]() {
+        for (;;) {
+            const uint32_t c = next.fetch_add(64u);
+            if (c >= count) { break; }
+            const uint32_t end = std::min(count, c + 64u);
+            for (uint32_t k = c; k < end; k++) {
+                const evo_genome& g = genomes[order[k]];
+                const Model md = build(g, *world);
+                fill(md, *world, records + rec_at[k], muscles + mus_at[k]);
+                const float period = g.muscle_count > 0u ? g.muscles[0].period : 0.0f;
+                uint32_t* h = heads + 8ull * k;
+                h[0] = g.node_count;
+                h[1] = g.muscle_count;
+                h[2] = evo::quake_hash(g.id);
+                h[3] = static_cast<uint32_t>(rec_at[k]);
+                h[4] = static_cast<uint32_t>(mus_at[k]);
+                h[5] = evo::as_uint(md.total_mass);
+                h[6] = evo::as_uint(md.inv_mass);
+                h[7] = (evo::f32_to_f16(period) & 0xffffu) | ((g.flags & 0xffffu) << 16u);
+            }
+        }
+    };
+    const int n = std::max(1, std::min(threads > 0 ? threads : static_cast<int>(std::thread::hardware_concurrency()), 64));
+    std::vector<std::thread> pool;
+    for (int t = 1; t < n; t++) { pool.emplace_back(work); }
+    work();
+    for (auto& t : pool) { t.join(); }
+    if (slots) {
+        for (uint32_t c = 0; c < count; c++) { slots[c] = order[c]; }
+    }
+    return EVO_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,98 @@
+"""Creatures' genes and their packing (include/evo_engine.h,
+evo_pack_genomes): the genome format of the reference's saves and fixtures
+(src/evolution.rs `Creature`, serialized as JSON) and the host packer that
+builds each body's model and start pose (src/physics2.rs:190-427) and fills
+the packed records (src/kernel.rs:336-593)."""
+from __future__ import annotations
+
+import ctypes as C
+import json
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _native as N
+from .batch import Batch
+
+NO_SENSOR = 255
+
+
+@dataclass
+class Creature:
+    """One creature's genes in canonical order (bone j joins node a, the head
+    or an earlier bone's child, to its own child b; bone 0 is the neck)."""
+    nodes: list[dict]
+    bones: list[dict]
+    muscles: list[dict]
+    id: int = 0
+    flags: int = 0
+    _keep: list = field(default_factory=list, repr=False)
+
+    @classmethod
+    def from_json(cls, text: str) -> "Creature":
+        d = json.loads(text)
+        return cls(d["nodes"], d["bones"], d["muscles"], int(d.get("id", 0)))
+
+    def to_c(self) -> N.Genome:
+        nodes = (N.NodeGene * len(self.nodes))(*[N.NodeGene(n["x"], n["y"], n["diameter"], n["friction"])
+                                                 for n in self.nodes])
+        bones = (N.BoneGene * max(len(self.bones), 1))(*[
+            N.BoneGene(b["a"], b["b"], b["rest_length"], b["min_angle"], b["max_angle"], b.get("organ_mass", 0.0),
+                       b.get("organ_at", 0.5)) for b in self.bones])
+        muscles = (N.MuscleGene * max(len(self.muscles), 1))(*[
+            N.MuscleGene(m["bone_a"], m["bone_b"], m["anchor_a"], m["anchor_b"], m["short"], m["long"], m["period"],
+                         m["phase"], m["duty"], m["stiffness"], m.get("sensor", NO_SENSOR), m.get("reset", 0.0),
+                         m.get("tendon", 0.0)) for m in self.muscles])
+        self._keep = [nodes, bones, muscles]
+        return N.Genome(C.cast(nodes, C.c_void_p), len(self.nodes), C.cast(bones, C.c_void_p), len(self.bones),
+                        C.cast(muscles, C.c_void_p), len(self.muscles), self.id & (2 ** 64 - 1), self.flags)
+
+
+def pack(creatures: list[Creature], world: N.WorldConfig, threads: int = 0) -> Batch:
+    """The creatures packed for a trial in `world`; batch.slots[c] is the
+    list index of packed creature c."""
+    arr = (N.Genome * len(creatures))(*[c.to_c() for c in creatures])
+    rw, mf = C.c_uint64(0), C.c_uint64(0)
+    lib = N.lib()
+    if lib.evo_pack_genome_sizes(arr, len(creatures), C.byref(rw), C.byref(mf)) != N.EVO_OK:
+        raise ValueError("a genome cannot be simulated (not a canonical tree, or too large)")
+    records = np.zeros(max(rw.value, 1), np.uint32)
+    muscles = np.zeros(max(mf.value, 4), np.float32)
+    heads = np.zeros((len(creatures), 8), np.uint32)
+    slots = np.zeros(len(creatures), np.uint32)
+    rc = lib.evo_pack_genomes(arr, len(creatures), C.byref(world), N.ptr(records), N.ptr(muscles), N.ptr(heads),
+                              N.ptr(slots), threads)
+    if rc != N.EVO_OK:
+        raise ValueError("packing failed")
+    return Batch(records, muscles, heads, slots)
+
+
+def random_creature(rng, nodes: int, muscles: int, id: int = 0) -> Creature:
+    """A random canonical creature: a random tree grown from the head, node
+    positions along its bones, muscles between two different bones."""
+    pos = [(0.0, 1.0)]
+    bones = []
+    for j in range(nodes - 1):
+        a = 0 if j == 0 else int(rng.integers(0, j + 1))
+        length = float(rng.uniform(0.2, 0.8))
+        ang = float(rng.uniform(-np.pi, np.pi))
+        pos.append((pos[a][0] + length * np.cos(ang), pos[a][1] + length * np.sin(ang)))
+        bones.append({"a": a, "b": j + 1, "rest_length": length, "min_angle": float(-rng.uniform(0.3, 2.0)),
+                      "max_angle": float(rng.uniform(0.3, 2.0)), "organ_mass": float(rng.choice([0.0, 0.02])),
+                      "organ_at": float(rng.uniform())})
+    ms = []
+    for _ in range(muscles):
+        ba = int(rng.integers(0, nodes - 1))
+        bb = int(rng.integers(0, nodes - 2)) if nodes > 2 else ba
+        if nodes > 2 and bb >= ba:
+            bb += 1
+        short = float(rng.uniform(0.1, 0.5))
+        ms.append({"bone_a": ba, "bone_b": bb, "anchor_a": float(rng.uniform()), "anchor_b": float(rng.uniform()),
+                   "short": short, "long": short + float(rng.uniform(0.0, 0.6)), "period": float(rng.uniform(0.3, 2.0)),
+                   "phase": float(rng.uniform()), "duty": float(rng.uniform(0.2, 0.8)),
+                   "stiffness": float(rng.uniform(5.0, 120.0)),
+                   "sensor": int(rng.choice([NO_SENSOR, 0, 1, 2, 3])), "reset": float(rng.uniform()),
+                   "tendon": float(rng.choice([0.0, 0.0, 0.5]))})
+    nodes_ = [{"x": float(x), "y": float(y), "diameter": float(rng.uniform(0.05, 0.15)),
+               "friction": float(rng.uniform(0.2, 1.0))} for x, y in pos]
+    return Creature(nodes_, bones, ms, id)

@@ -276,6 +276,52 @@ int evo_describe_device(const uint32_t* d_records, const uint32_t* d_heads, cons
                         uint32_t base, uint32_t count, uint32_t steps, uint32_t rate, uint32_t layout,
                         evo_metrics* d_out, void* stream);
 
+/* ---------------------------------------------------------------- genomes */
+
+/* A creature's genes (src/evolution.rs Creature, NodeGene, Bone, Muscle),
+ * in canonical order: bone j joins node a (the head, node 0, for bone 0, the
+ * neck; otherwise the head or the child of an earlier bone) to its own child
+ * node b. A muscle's sensor is 0-3 (an end of its two bones: bone_a.a,
+ * bone_a.b, bone_b.a, bone_b.b) or 255 for none. */
+typedef struct evo_node_gene {
+    float x, y, diameter, friction;
+} evo_node_gene;
+typedef struct evo_bone_gene {
+    uint32_t a, b;
+    float rest_length, min_angle, max_angle, organ_mass, organ_at;
+} evo_bone_gene;
+typedef struct evo_muscle_gene {
+    uint32_t bone_a, bone_b;
+    float anchor_a, anchor_b, shortest, longest, period, phase, duty, stiffness;
+    uint32_t sensor;
+    float reset, tendon;
+} evo_muscle_gene;
+typedef struct evo_genome {
+    const evo_node_gene* nodes;
+    uint32_t node_count;
+    const evo_bone_gene* bones;
+    uint32_t bone_count;
+    const evo_muscle_gene* muscles;
+    uint32_t muscle_count;
+    uint64_t id;
+    uint32_t flags; /* EVO_FLAG_* of the early rungs */
+} evo_genome;
+
+/* Buffer sizes of evo_pack_genomes for these genomes; EVO_ERR_INVALID if one
+ * cannot be simulated. */
+int evo_pack_genome_sizes(const evo_genome* genomes, uint32_t count, uint64_t* record_words, uint64_t* muscle_floats);
+
+/* Packs genomes for a trial in `world` (kernel::pack_reusing,
+ * src/kernel.rs:471-593): sorted by (muscles, nodes, index), each body's
+ * model built (masses of nodes, bones, organs and muscles; joint ranges
+ * around the rest angles; muscle strengths from the mass they drive,
+ * src/physics2.rs:190-342) and its start pose put with its centre of mass
+ * over x = 0 and its lowest surface on the ground (src/physics2.rs:365-398).
+ * slots[c] receives the genome index of packed creature c. `threads` host
+ * threads fill creatures (0: one per CPU). */
+int evo_pack_genomes(const evo_genome* genomes, uint32_t count, const evo_world_config* world, uint32_t* records,
+             float* muscles, uint32_t* heads, uint32_t* slots, int threads);
+
 /* ---------------------------------------------------------------- engine */
 
 /* A submission engine on one GPU: the device side of the reference's

@@ -1,0 +1,180 @@
+"""Genome packing (src/physics.rs:182-296, src/physics2.rs:190-427,
+src/kernel.rs:336-593) on the reference's own fixture
+(tests/fixtures/warp_walker.json, copied to tests/golden/) and on random
+canonical creatures, pinned by what the model must conserve: mass, bone
+lengths, rest angles, the start pose's centre and ground contact, the muscle
+record's derived fields, the packing order and the slot map."""
+import numpy as np
+import pytest
+
+import evolution_simulator_gpu_amd as E
+import oracle
+from evolution_simulator_gpu_amd import _native as N
+from evolution_simulator_gpu_amd import genome as G
+from tests.helpers import assert_bit_exact
+
+from pathlib import Path
+
+WALKER = (Path(__file__).resolve().parent / "golden" / "warp_walker.json").read_text()
+
+
+def _walker(id=0):
+    c = G.Creature.from_json(WALKER)
+    c.id = id
+    return c
+
+
+def _unpack(batch, c):
+    h = batch.heads[c]
+    n, m = int(h[0] & 255), int(h[1])
+    rec = batch.records[h[3]:h[3] + n * 6].reshape(n, 6)
+    bones = batch.records[h[3] + n * 6:h[3] + n * 6 + (n - 1) * 6].reshape(n - 1, 6)
+    mus = batch.muscles[h[4]:h[4] + m * 20].reshape(m, 20)
+    return rec, bones, mus
+
+
+def _order(creature):
+    return [0] + [b["b"] for b in creature.bones]
+
+
+def test_walker_mass_is_conserved():
+    """Node masses (0.1 kg at 8 cm, with the square of the diameter), half of
+    each bone's 4 kg/m^2 * L^2 at either end, organs, and 0.05 kg + 1 kg/m
+    of slack length per muscle: the packed nodes carry the sum."""
+    w = _walker()
+    batch = G.pack([w], E.world())
+    rec, _, mus = _unpack(batch, 0)
+    d = np.array([n["diameter"] for n in w.nodes])
+    nodes = np.clip(0.1 * (d / 0.08) ** 2, 0.02, 10.0).sum()
+    bones = sum(4.0 * b["rest_length"] ** 2 + b["organ_mass"] for b in w.bones)
+    muscles = sum(0.05 + float(mus[k, 7]) for k in range(len(w.muscles)))
+    total = rec[:, 0].view(np.float32).astype(np.float64).sum()
+    assert abs(total - (nodes + bones + muscles)) < 1e-5 * total
+    assert batch.heads[0, 5:6].view(np.float32)[0] == np.float32(total) or \
+        abs(batch.heads[0, 5:6].view(np.float32)[0] - total) < 1e-5 * total
+    # slack: the longest length or the start span, whichever is longer
+    assert np.all(mus[:, 7] >= np.array([m["long"] for m in w.muscles], np.float32))
+
+
+@pytest.mark.parametrize("world", [dict(), dict(terrain=0.08), dict(quake=0.12, terrain=0.03), dict(ground=0)])
+def test_walker_start_pose(world):
+    """Bones keep their rest lengths, joints their rest angles; the centre
+    of mass sits over x = 0 and the lowest surface on the ground
+    (src/physics2.rs:365-398)."""
+    w = _walker(id=77)
+    cfg = E.world(**world)
+    batch = G.pack([w], cfg)
+    rec, bones, _ = _unpack(batch, 0)
+    x = rec[:, 3].view(np.float32).astype(np.float64)
+    y = rec[:, 4].view(np.float32).astype(np.float64)
+    m = rec[:, 0].view(np.float32).astype(np.float64)
+    r = rec[:, 1].view(np.float32).astype(np.float64)
+    assert abs((m * x).sum() / m.sum()) < 1e-6
+    pivot = bones[:, 0]
+    length = bones[:, 1].view(np.float32)
+    for j in range(len(pivot)):
+        assert abs(np.hypot(x[j + 1] - x[pivot[j]], y[j + 1] - y[pivot[j]]) - length[j]) < 1e-5
+    # absolute bone angles from the pose; relative to the parent bone they are
+    # the rest angles (low end of the range minus the gene's min angle)
+    ang = np.arctan2(y[1:] - y[pivot], x[1:] - x[pivot])
+    lo = bones[:, 3].view(np.float32)
+    genes = w.bones
+    order = _order(w)
+    for j in range(1, len(pivot)):
+        parent = bones[j, 2]
+        rest = float(lo[j]) - genes[j]["min_angle"]
+        rel = (ang[j] - ang[parent] + np.pi) % (2 * np.pi) - np.pi
+        assert abs(((rel - rest + np.pi) % (2 * np.pi)) - np.pi) < 1e-5
+    if world.get("ground", 1):
+        lib = N.lib()
+        amp = cfg.terrain + (cfg.quake * (0.6 + ((lib.evo_quake_hash(77) >> 16) & 0xFFFF) * (0.8 / 65536))
+                             if cfg.quake > 0 else 0.0)
+        phase = (lib.evo_quake_hash(77) & 0xFFFF) / 65536.0 if cfg.quake > 0 else 0.0
+        h = np.array([lib.evo_ground_height(cfg, float(xi), amp, phase) for xi in x])
+        assert abs((y - r - h).min()) < 1e-5
+    else:
+        assert abs(y[0] - w.nodes[0]["y"]) < 1e-6  # not lowered without a ground
+    assert len(order) == len(w.nodes)
+
+
+def test_walker_feet_and_muscle_records():
+    """The one foot ends the leg (src/kernel.rs:345-351); muscle records
+    carry the genes' rhythm, the renumbered sensor node and the strength
+    bounds (src/kernel.rs:370-409)."""
+    w = _walker()
+    batch = G.pack([w], E.world())
+    rec, _, mus = _unpack(batch, 0)
+    assert list(rec[:, 5]) == [0, 0, 0, 0, 0, 1]
+    order = _order(w)
+    renumber = {node: r for r, node in enumerate(order)}
+    f32 = np.float32
+    for k, g in enumerate(w.muscles):
+        row = mus[k]
+        assert row[9] == f32(1.0) / f32(g["period"])
+        assert row[10] == f32(g["phase"]) and row[11] == f32(g["duty"]) and row[14] == f32(g["stiffness"])
+        cap = f32(f32(f32(f32(48.0) * f32(g["period"])) * min(f32(g["duty"]), f32(1.0) - f32(g["duty"])))) / f32(np.pi)
+        assert row[8] == min(f32(g["long"]) - f32(g["short"]), cap)
+        strength = row[4] / 200.0
+        assert 0.0 < strength <= 1.0
+        assert abs(row[5] - 1.0 / (120.0 * strength)) < 1e-6 * row[5]
+        assert row[3] == f32(1.0) / (f32(8.0) * max(f32(g["long"]), f32(0.05)))
+        sensor = row[16:17].view(np.uint32)[0]
+        if g["sensor"] == 255:
+            assert sensor == N.NONE
+        else:
+            ba, bb = w.bones[g["bone_a"]], w.bones[g["bone_b"]]
+            assert sensor == renumber[[ba["a"], ba["b"], bb["a"], bb["b"]][g["sensor"]]]
+        nodes = row[0:1].view(np.uint32)[0]
+        assert (nodes >> 8) & 255 == g["bone_a"] + 1 and nodes >> 24 == g["bone_b"] + 1
+    # muscles on one pair of bones on one side share their strength: at most
+    # as strong together as one alone could be
+    same = {}
+    for k, g in enumerate(w.muscles):
+        same.setdefault((min(g["bone_a"], g["bone_b"]), max(g["bone_a"], g["bone_b"])), []).append(mus[k, 4])
+    assert any(len(v) > 1 for v in same.values())
+
+
+def test_random_creatures_pack_once_in_order():
+    """src/kernel.rs:643-656: every creature packs once; packed order is by
+    (muscles, nodes, index) and the slots map back."""
+    rng = np.random.default_rng(3)
+    cs = [G.random_creature(rng, int(rng.integers(2, 14)), int(rng.integers(0, 20)), id=i) for i in range(300)]
+    batch = G.pack(cs, E.world(), threads=4)
+    batch.validate()
+    assert np.array_equal(np.sort(batch.slots), np.arange(300))
+    key = batch.muscle_counts().astype(np.int64) * 1000 + batch.nodes() * 1 + 0
+    assert np.all(np.diff(key) >= 0)
+    for c in range(0, 300, 37):
+        src = cs[batch.slots[c]]
+        assert batch.nodes()[c] == len(src.nodes) and batch.muscle_counts()[c] == len(src.muscles)
+        assert batch.heads[c, 2] == N.lib().evo_quake_hash(src.id)
+    again = G.pack(cs, E.world(), threads=1)
+    assert np.array_equal(again.records, batch.records) and np.array_equal(again.heads, batch.heads)
+
+
+def test_non_canonical_genomes_are_refused():
+    w = _walker()
+    bad = G.Creature(w.nodes, [dict(b) for b in w.bones], w.muscles)
+    bad.bones[2]["a"] = 4  # node 4 is reached later: not parent-first
+    with pytest.raises(ValueError):
+        G.pack([bad], E.world())
+    loop = G.Creature(w.nodes, [dict(b) for b in w.bones], w.muscles)
+    loop.bones[4]["b"] = 1  # a node reached twice
+    with pytest.raises(ValueError):
+        G.pack([loop], E.world())
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("world", [dict(duration=4.0), dict(duration=4.0, quake=0.05, terrain=0.03)])
+def test_packed_walkers_and_random_bodies_on_the_gpu(gpu, world):
+    """Eight walkers with their own ids (tests/cuda_effects.rs:31-43) and
+    random bodies, packed here, run bit for bit as the oracle runs them."""
+    rng = np.random.default_rng(5)
+    cs = [_walker(id=i) for i in range(8)]
+    cs += [G.random_creature(rng, int(rng.integers(2, 20)), int(rng.integers(0, 40)), id=100 + i) for i in range(200)]
+    w = E.world(substeps=8, **world)
+    batch = G.pack(cs, w)
+    p, run = E.run_for(batch, w)
+    want, _, _ = oracle.run_f32(batch, p, run, threads=8)
+    got = E.simulate(batch, w)
+    assert_bit_exact(got, want, "packed genomes")
