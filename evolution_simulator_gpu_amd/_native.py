@@ -204,6 +204,7 @@ SIGNATURES = {
     "evo_last_error": ([_P], C.c_char_p),
     "evo_pack_genome_sizes": ([_P, _u32, C.POINTER(_u64), C.POINTER(_u64)], C.c_int),
     "evo_pack_genomes": ([_P, _u32, C.POINTER(WorldConfig), _P, _P, _P, _P, C.c_int], C.c_int),
+    "evo_decode_recording": ([_P, _u64, _u32, _u32, _P, _u32, _u32, _u32, _P, _P, _P, _P, _P, _P], C.c_int64),
     "evo_validate_batch": ([_P, _u64, _P, _u64, _P, _u32, C.POINTER(_u32)], C.c_int),
     "evo_max_muscles": ([_P, _u32, _u32], _u32),
     "evo_build_info": ([], C.c_char_p),

@@ -96,3 +96,30 @@ def random_creature(rng, nodes: int, muscles: int, id: int = 0) -> Creature:
     nodes_ = [{"x": float(x), "y": float(y), "diameter": float(rng.uniform(0.05, 0.15)),
                "friction": float(rng.uniform(0.2, 1.0))} for x, y in pos]
     return Creature(nodes_, bones, ms, id)
+
+
+def node_order(creature: Creature) -> np.ndarray:
+    """The creature node that each kernel node is: the head, then each
+    bone's child (src/physics2.rs:196-199)."""
+    return np.array([0] + [b["b"] for b in creature.bones], np.uint32)
+
+
+def decode_recording(frames: np.ndarray, stride: int, creature: Creature, every: int = 1) -> dict:
+    """One creature's recorded frames (as evo_record writes them) decoded in
+    the creature's own node numbering, every `every`-th frame and the last:
+    positions, muscle energy and pull, ground push and friction per node and
+    the broken-joint bits (engine::recorded and engine::thin)."""
+    frames = np.ascontiguousarray(frames, np.float32)
+    count = frames.size // (2 * stride)
+    n, m = len(creature.nodes), len(creature.muscles)
+    order = node_order(creature)
+    kept = (count + every - 1) // every + 1
+    out = {"positions": np.zeros((kept, n, 2), np.float32), "energy": np.zeros((kept, m), np.float32),
+           "pull": np.zeros((kept, m), np.float32), "ground": np.zeros((kept, n), np.float32),
+           "friction": np.zeros((kept, n), np.float32), "broken": np.zeros(kept, np.uint64)}
+    k = N.lib().evo_decode_recording(frames.ctypes.data, frames.size, stride, count, order.ctypes.data, n, m, every,
+                                     *(out[key].ctypes.data for key in ("positions", "energy", "pull", "ground",
+                                                                        "friction", "broken")))
+    if k < 0:
+        raise ValueError("frames do not match the creature")
+    return {key: v[:k] for key, v in out.items()}

@@ -322,6 +322,18 @@ int evo_pack_genome_sizes(const evo_genome* genomes, uint32_t count, uint64_t* r
 int evo_pack_genomes(const evo_genome* genomes, uint32_t count, const evo_world_config* world, uint32_t* records,
              float* muscles, uint32_t* heads, uint32_t* slots, int threads);
 
+/* One creature's recording decoded for viewing (engine::recorded and
+ * engine::thin, src/engine.rs:1062-1127, :127-152): of its `count` frames of
+ * `stride` float pairs, every `every`-th is kept, and the last; node values go
+ * back to the creature's own numbering (kernel node k is creature node
+ * order[k]). Per kept frame: positions (nodes x 2), muscle energy and pull
+ * (muscles each), ground push and friction (nodes each), the broken-joint
+ * bits. Any output may be NULL. Returns the frames kept, or EVO_ERR_INVALID. */
+int64_t evo_decode_recording(const float* frames, uint64_t frame_floats, uint32_t stride, uint32_t count,
+                             const uint32_t* order, uint32_t nodes, uint32_t muscles, uint32_t every,
+                             float* positions, float* energy, float* pull, float* ground, float* friction,
+                             uint64_t* broken);
+
 /* ---------------------------------------------------------------- engine */
 
 /* A submission engine on one GPU: the device side of the reference's

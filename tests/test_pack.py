@@ -178,3 +178,41 @@ def test_packed_walkers_and_random_bodies_on_the_gpu(gpu, world):
     want, _, _ = oracle.run_f32(batch, p, run, threads=8)
     got = E.simulate(batch, w)
     assert_bit_exact(got, want, "packed genomes")
+
+
+def test_recording_decodes_to_the_creature_numbering():
+    """engine::recorded / engine::thin (src/engine.rs:1062-1127, :127-152) on
+    the oracle's recording of the walker: frame 0 is the packed start pose in
+    the creature's own node numbering, bones keep close to their lengths,
+    energies stay in [0, 1], the last frame's centre of mass is the score,
+    and thinning keeps every n-th frame and the last."""
+    w = _walker(id=3)
+    cfg = E.world(duration=2.0, substeps=8)
+    batch = G.pack([w], cfg)
+    p, run = E.run_for(batch, cfg, record=True)
+    res, frames, _ = oracle.run_f32(batch, p, run, frames=True)
+    full = G.decode_recording(frames, p.stride, w)
+    count = frames.size // (2 * p.stride)
+    assert full["positions"].shape == (count, len(w.nodes), 2)
+    rec, _, _ = _unpack(batch, 0)
+    order = G.node_order(w)
+    start = np.stack([rec[:, 3].view(np.float32), rec[:, 4].view(np.float32)], axis=1)
+    assert np.array_equal(full["positions"][0][order], start)
+    pos = full["positions"].astype(np.float64)
+    for b in w.bones:
+        d = np.hypot(*(pos[:, b["b"]] - pos[:, b["a"]]).T)
+        assert np.abs(d - b["rest_length"]).max() < 0.05 * b["rest_length"] + 1e-3
+    assert full["energy"].min() >= 0.0 and full["energy"].max() <= 1.0
+    mass = rec[:, 0].view(np.float32).astype(np.float64)
+    # the score is the centre of mass after the trial's last step (at a fall
+    # the recording goes on limp): frame settle + steps
+    steps = int(res["ground_hi"].view(np.uint32)[0] >> 16)
+    at = N.lib().evo_settle_frames(cfg.rate) + steps
+    com = (pos[at][order][:, 0] * mass).sum() / mass.sum()
+    assert abs(com - res["fitness"][0]) < 1e-5
+    thin = G.decode_recording(frames, p.stride, w, every=7)
+    keep = [t for t in range(count) if t % 7 == 0 or t == count - 1]
+    assert len(thin["positions"]) == len(keep)
+    for key in ("positions", "energy", "ground", "friction", "broken"):
+        assert np.array_equal(thin[key], full[key][keep])
+    assert np.all(full["ground"] >= 0.0)
