@@ -58,10 +58,11 @@ def _run(cmd: list[str]) -> str:
 def _jobs() -> list[tuple[list[str], Path]]:
     hipcc = _hipcc()
     jobs = []
-    for rate, sub in RATES_SUBSTEPS:
-        out = OBJ / f"instances_r{rate}_s{sub}.o"
-        jobs.append(([hipcc, *HIP_FLAGS, f"-DEVO_RATE={rate}", f"-DEVO_SUB={sub}", "-c",
-                      str(CSRC / "instances.hip"), "-o", str(out)], out))
+    for stem in ("instances", "adjoint_instances"):
+        for rate, sub in RATES_SUBSTEPS:
+            out = OBJ / f"{stem}_r{rate}_s{sub}.o"
+            jobs.append(([hipcc, *HIP_FLAGS, f"-DEVO_RATE={rate}", f"-DEVO_SUB={sub}", "-c",
+                          str(CSRC / f"{stem}.hip"), "-o", str(out)], out))
     for src in HIP_SOURCES:
         out = OBJ / (Path(src).stem + ".o")
         jobs.append(([hipcc, *HIP_FLAGS, "-c", str(CSRC / src), "-o", str(out)], out))

@@ -1,89 +1,29 @@
-// The adjoint launch: d(fitness)/d(parameters) of every creature of a batch,
-// one creature per lane, from the functions of adjoint.hpp.
-//
-// Memory: every `seg` steps a checkpoint of the state (positions,
-// velocities, energy stores) goes to the workspace; going backwards, a
-// segment is recomputed from its checkpoint with the start state of each of
-// its substeps written to the workspace, then reversed substep by substep.
-// Workspace slots are laid out [slot][word][creature], so the lanes of a
-// wave read and write consecutive words.
+// The adjoint entry points: argument checks, the workspace layout and the
+// choice of kernel instance (adjoint_instances.hip, adjoint_kernel.hpp).
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
 
 #include "../../include/evo_engine.h"
-#include "adjoint.hpp"
+#include "adjoint_kernel.hpp"
+#include "adjoint_launch.hpp"
 
 namespace {
 
-using evo::adj::Adj;
-using evo::adj::Body;
-using evo::adj::World;
+using evo::dev::adj::AdjointArgs;
+using evo::dev::adj::AdjointLauncher;
+using Large = evo::dev::adj::Words<evo::dev::adj::kLargeNodes, evo::dev::adj::kLargeMuscles>;
 
-constexpr int kN = EVO_ADJ_MAX_NODES;
-constexpr int kM = EVO_ADJ_MAX_MUSCLES;
-constexpr uint32_t kWords = 4u * kN + kM;
+uint32_t segments(uint32_t steps, uint32_t seg) { return (steps + seg - 1u) / seg; }
+uint64_t lanes(uint32_t count) { return (static_cast<uint64_t>(count) + 63u) & ~static_cast<uint64_t>(63u); }
 
-// The workspace of one launch: checkpoints, then the per-substep tape.
-struct DeviceIO {
-    float* ckpt;
-    float* tape;
-    uint32_t count, lane;
-    __device__ void write(float* base, uint32_t slot, const Body<kN, kM>& b) const {
-        float* p = base + static_cast<uint64_t>(slot) * kWords * count + lane;
-        for (uint32_t i = 0; i < b.nodes; i++) {
-            p[static_cast<uint64_t>(i) * count] = b.px[i];
-            p[static_cast<uint64_t>(kN + i) * count] = b.py[i];
-            p[static_cast<uint64_t>(2 * kN + i) * count] = b.vx[i];
-            p[static_cast<uint64_t>(3 * kN + i) * count] = b.vy[i];
-        }
-        for (uint32_t k = 0; k < b.muscles; k++) { p[static_cast<uint64_t>(4 * kN + k) * count] = b.store[k]; }
-    }
-    __device__ void read(const float* base, uint32_t slot, Body<kN, kM>& b) const {
-        const float* p = base + static_cast<uint64_t>(slot) * kWords * count + lane;
-        for (uint32_t i = 0; i < b.nodes; i++) {
-            b.px[i] = p[static_cast<uint64_t>(i) * count];
-            b.py[i] = p[static_cast<uint64_t>(kN + i) * count];
-            b.vx[i] = p[static_cast<uint64_t>(2 * kN + i) * count];
-            b.vy[i] = p[static_cast<uint64_t>(3 * kN + i) * count];
-        }
-        for (uint32_t k = 0; k < b.muscles; k++) { b.store[k] = p[static_cast<uint64_t>(4 * kN + k) * count]; }
-    }
-    __device__ void ckpt_put(uint32_t g, const Body<kN, kM>& b) { write(ckpt, g, b); }
-    __device__ void ckpt_get(uint32_t g, Body<kN, kM>& b) { read(ckpt, g, b); }
-    __device__ void put(uint32_t s, const Body<kN, kM>& b) { write(tape, s, b); }
-    __device__ void get(uint32_t s, Body<kN, kM>& b) { read(tape, s, b); }
-};
-
-__global__ void __launch_bounds__(64) adjoint_kernel(const uint32_t* __restrict__ records,
-                                                     const float* __restrict__ muscles,
-                                                     const uint32_t* __restrict__ heads,
-                                                     const evo_result* __restrict__ results, const evo_params p,
-                                                     uint32_t flags, uint32_t rate, uint32_t substeps, uint32_t seg,
-                                                     float* __restrict__ grads, float* __restrict__ fitness,
-                                                     float* __restrict__ ckpt, float* __restrict__ tape) {
-    const uint32_t lane = blockIdx.x * blockDim.x + threadIdx.x;
-    if (lane >= p.count) { return; }
-    const uint32_t c = p.base + lane;
-    const World w = evo::adj::make_world(p, flags, rate, substeps);
-    Body<kN, kM> b;
-    Adj<kN, kM> a;
-    float* out = grads + static_cast<uint64_t>(lane) * EVO_ADJ_STRIDE;
-    const uint32_t code = __float_as_uint(results[c].ground_hi);
-    // the trial's own length, never past the launch's (the workspace's size)
-    const uint32_t steps = min(code >> 16u, p.steps);
-    if (!evo::adj::load(b, w, c, records, muscles, heads) || (code & 32u) != 0u || steps == 0u) {
-        // a failed trial scores a constant: its gradient is zero
-        for (uint32_t q = 0; q < EVO_ADJ_STRIDE; q++) { out[q] = 0.0f; }
-        fitness[lane] = results[c].fitness;
-        return;
-    }
-    DeviceIO io{ckpt, tape, p.count, lane};
-    fitness[lane] = evo::adj::trial_gradient(w, b, steps, substeps, seg, io, a);
-    evo::adj::store_gradient(b, a, out, EVO_ADJ_STRIDE);
+AdjointLauncher pick(const evo_run& run) {
+    if (run.rate == 60u && run.substeps == 8u) { return evo::dev::adj::adjoint_pick_r60_s8(run); }
+    if (run.rate == 60u && run.substeps == 16u) { return evo::dev::adj::adjoint_pick_r60_s16(run); }
+    if (run.rate == 120u && run.substeps == 8u) { return evo::dev::adj::adjoint_pick_r120_s8(run); }
+    if (run.rate == 120u && run.substeps == 16u) { return evo::dev::adj::adjoint_pick_r120_s16(run); }
+    return nullptr;
 }
-
-uint32_t max_segments(uint32_t steps, uint32_t seg) { return (steps + seg - 1u) / seg; }
 
 }  // namespace
 
@@ -91,8 +31,11 @@ extern "C" {
 
 uint64_t evo_adjoint_workspace_bytes(uint32_t count, uint32_t steps, uint32_t substeps, uint32_t seg) {
     if (seg == 0u) { return 0u; }
-    const uint64_t slots = static_cast<uint64_t>(max_segments(steps, seg)) + static_cast<uint64_t>(seg) * substeps;
-    return slots * kWords * count * sizeof(float);
+    // the largest class's records bound every class's
+    const uint64_t words = static_cast<uint64_t>(segments(steps, seg)) * Large::ckpt +
+                           static_cast<uint64_t>(seg) * substeps * Large::slot +
+                           static_cast<uint64_t>(seg) * Large::step + Large::columns;
+    return words * lanes(count) * sizeof(float);
 }
 
 int evo_adjoint_device(const uint32_t* d_records, const float* d_muscles, const uint32_t* d_heads,
@@ -101,25 +44,36 @@ int evo_adjoint_device(const uint32_t* d_records, const float* d_muscles, const 
     if (!d_records || !d_muscles || !d_heads || !d_results || !p || !run || !d_grads || !d_fitness || !d_workspace) {
         return EVO_ERR_INVALID;
     }
-    if (seg == 0u || p->steps == 0u || run->record || (run->flags & ~evo::adj::kAdjointFlags) != 0u) {
+    if (seg == 0u || p->steps == 0u || run->record || (run->flags & ~evo::dev::adj::kFlags) != 0u) {
         return EVO_ERR_INVALID;
     }
-    if (run->max_nodes == 0u || run->max_nodes > static_cast<uint32_t>(kN) ||
-        run->max_muscles > static_cast<uint32_t>(kM)) {
+    if (run->max_nodes == 0u || run->max_nodes > static_cast<uint32_t>(EVO_ADJ_MAX_NODES) ||
+        run->max_muscles > static_cast<uint32_t>(EVO_ADJ_MAX_MUSCLES)) {
         return EVO_ERR_INVALID;
     }
-    if (run->substeps != 8u && run->substeps != 16u) { return EVO_ERR_INVALID; }
+    const AdjointLauncher launch = pick(*run);
+    if (!launch) { return EVO_ERR_INVALID; }
     if (workspace_bytes < evo_adjoint_workspace_bytes(p->count, p->steps, run->substeps, seg)) {
         return EVO_ERR_INVALID;
     }
     if (p->count == 0u) { return EVO_OK; }
-    float* ckpt = static_cast<float*>(d_workspace);
-    float* tape = ckpt + static_cast<uint64_t>(max_segments(p->steps, seg)) * kWords * p->count;
-    const uint32_t blocks = (p->count + 63u) / 64u;
-    hipLaunchKernelGGL(adjoint_kernel, dim3(blocks), dim3(64), 0, reinterpret_cast<hipStream_t>(stream), d_records,
-                       d_muscles, d_heads, d_results, *p, run->flags, run->rate, run->substeps, seg, d_grads,
-                       d_fitness, ckpt, tape);
-    const hipError_t e = hipGetLastError();
+    if ((reinterpret_cast<uintptr_t>(d_workspace) & 15u) != 0u || (reinterpret_cast<uintptr_t>(d_heads) & 15u) != 0u) {
+        return EVO_ERR_INVALID;
+    }
+    AdjointArgs x;
+    x.records = d_records;
+    x.muscles = d_muscles;
+    x.heads = reinterpret_cast<const uint4*>(d_heads);
+    x.results = d_results;
+    x.p = *p;
+    x.flags = run->flags;
+    x.seg = seg;
+    x.segments = segments(p->steps, seg);
+    x.grads = d_grads;
+    x.fitness = d_fitness;
+    x.workspace = static_cast<float*>(d_workspace);
+    x.stride = lanes(p->count);
+    const hipError_t e = launch(x, reinterpret_cast<hipStream_t>(stream));
     if (e == hipErrorOutOfMemory) { return EVO_ERR_OOM; }
     return e == hipSuccess ? EVO_OK : EVO_ERR_DEVICE;
 }

@@ -1,0 +1,59 @@
+// The adjoint kernel instances of one (steps per second, substeps) pair,
+// chosen by EVO_RATE and EVO_SUB at compile time: per world (the calm flat
+// world, or any world of the differentiated effects read from a scalar
+// register) a class for bodies of up to 10 nodes and 16 muscles without
+// tendons, and one for up to 12 nodes and 24 muscles.
+#include <hip/hip_runtime.h>
+
+#include "adjoint_kernel.hpp"
+#include "adjoint_launch.hpp"
+
+#ifndef EVO_RATE
+#error "EVO_RATE must be defined"
+#endif
+#ifndef EVO_SUB
+#error "EVO_SUB must be defined"
+#endif
+
+#define EVO_CAT2(a, b, c, d) a##b##c##d
+#define EVO_CAT(a, b, c, d) EVO_CAT2(a, b, c, d)
+#define EVO_ADJ_PICK EVO_CAT(adjoint_pick_r, EVO_RATE, _s, EVO_SUB)
+
+namespace evo {
+namespace dev {
+namespace adj {
+namespace {
+
+template <class C>
+hipError_t launch(const AdjointArgs& x, hipStream_t stream) {
+    using W = Words<C::MAXN, C::MAXM>;
+    const uint64_t stride = x.stride;
+    float* ckpt = x.workspace;
+    float* slots = ckpt + static_cast<uint64_t>(x.segments) * W::ckpt * stride;
+    float* steps = slots + static_cast<uint64_t>(x.seg) * C::substeps * W::slot * stride;
+    float* rhythm = steps + static_cast<uint64_t>(x.seg) * W::step * stride;
+    const uint32_t blocks = (x.p.count + 63u) / 64u;
+    hipLaunchKernelGGL(adjoint_kernel<C>, dim3(blocks), dim3(64), 0, stream, x.records, x.muscles, x.heads,
+                       x.results, x.p, x.flags, x.seg, x.grads, x.fitness, ckpt, slots, steps, rhythm, stride);
+    return hipGetLastError();
+}
+
+template <uint32_t WORLD>
+AdjointLauncher pick_world(const evo_run& run) {
+    constexpr uint32_t R = EVO_RATE, S = EVO_SUB;
+    if (run.no_tendons && run.max_nodes <= static_cast<uint32_t>(kSmallNodes) &&
+        run.max_muscles <= static_cast<uint32_t>(kSmallMuscles)) {
+        return &launch<Cfg<R, S, WORLD, kSmallNodes, kSmallMuscles, false, 64, true, false>>;
+    }
+    return &launch<Cfg<R, S, WORLD, kLargeNodes, kLargeMuscles, false, 64, true, true>>;
+}
+
+}  // namespace
+
+AdjointLauncher EVO_ADJ_PICK(const evo_run& run) {
+    return run.flags == EVO_GROUND ? pick_world<EVO_GROUND>(run) : pick_world<kAnyWorld>(run);
+}
+
+}  // namespace adj
+}  // namespace dev
+}  // namespace evo

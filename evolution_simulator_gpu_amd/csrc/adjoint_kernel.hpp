@@ -1,0 +1,1107 @@
+// Reverse-mode derivatives of creature trials on gfx950: d(fitness)/d(muscle
+// rhythm amplitude, inverse period, phase, Hill factor, force cap, inverse
+// energy capacity; node friction), one creature per lane.
+//
+// The reference has no adjoint; this differentiates its trial physics
+// (shaders/creature.cu:442-742 and the sensor resets of :956-1000). The
+// forward passes are the simulator's own stage functions (sim_kernel.hpp),
+// so every recomputed state is the simulated state bit for bit; the reverse
+// stages are the transposed Jacobians of those stages, derived by hand.
+// Discrete decisions (ground contact, a joint outside its range, the clamps
+// of the Hill factor, force cap and energy store, the friction cone, sensor
+// resets) are taken as the forward pass took them, so the result is the
+// derivative of the piecewise-smooth trajectory the forward pass followed.
+// At a tie of min(a, b) or max(a, b) the derivative follows a (the value
+// expression written first), and clamp(x, lo, hi) = min(max(x, lo), hi)
+// passes it to x on [lo, hi] closed: so a muscle of zero amplitude or a
+// store at exactly full charge still carries a derivative.
+//
+// Layout on the chip, as in the forward kernel: the creature's constants,
+// the muscles' stores and their adjoints, and the late half of one
+// substep's tape (joints, ground, damping) live in registers (every loop
+// unrolled to the class's maximum body); the node state (position and
+// velocity), the force-then-previous-position pairs, the node adjoints and
+// the early half of the tape (drag, bone lengths) live in LDS, one column per
+// lane. The parameter gradient sums are only ever added to: they live in
+// device memory, one column per lane, and take fire-and-forget atomic adds
+// (each address is its own lane's, so the adds land in program order and the
+// sums are reproducible). The rhythm offsets and reset times, touched once a
+// step, live there too.
+//
+// Memory: a checkpoint of the state every `seg` steps; going backwards, a
+// segment is recomputed from its checkpoint with the start state of every
+// substep written to the workspace, and each substep is recomputed once more
+// from there with its tape and reversed. Workspace records are laid out
+// [record][word][lane], so a wave's lanes touch consecutive words.
+#pragma once
+
+#include "../../include/evo_engine.h"
+#include "sim_kernel.hpp"
+
+namespace evo {
+namespace dev {
+namespace adj {
+
+// The effects the adjoint differentiates.
+constexpr uint32_t kFlags = EVO_GROUND | EVO_TERRAIN | EVO_SLOPE | EVO_GAPS | EVO_HURDLES | EVO_QUAKE | EVO_ICE |
+                            EVO_WIND | EVO_AIR;
+
+// Workspace words per lane: a checkpoint (node state, stores, rhythm
+// offsets, reset times, ground bits), a substep's start state (node state,
+// stores), a step's rhythm offsets and reset times.
+template <int MAXN, int MAXM>
+struct Words {
+    static constexpr uint32_t ckpt = 4u * MAXN + 3u * MAXM + 1u;
+    static constexpr uint32_t slot = 4u * MAXN + MAXM;
+    static constexpr uint32_t step = 2u * MAXM;
+    // the lane's columns: rhythm offsets and reset times, then the gradient
+    // sums (six per muscle in gradient order, then the node frictions)
+    static constexpr uint32_t columns = 2u * MAXM + 6u * MAXM + MAXN;
+};
+
+// One substep's tape: what the reverse needs beyond the substep's start state
+// and its end state. The early half, read last, sits in the lane's LDS
+// column `e`: per bone the mean velocity of its nodes before its drag, then
+// per bone node 1 - node 0 before its projection.
+template <class C>
+struct Tape {
+    float* e;
+    float jux[C::MAXB], juy[C::MAXB], jvx[C::MAXB], jvy[C::MAXB], jerr[C::MAXB];  // joint j before its push
+    float gx[C::MAXN], gy[C::MAXN];    // node i before the ground
+    float dlam[C::MAXB];               // joint j's damping multiplier
+    static constexpr int kWords = 4 * C::MAXB;
+    __device__ __forceinline__ float& dw(int j, int c) const { return e[(2 * j + c) * C::BLOCK]; }
+    __device__ __forceinline__ float& bd(int j, int c) const { return e[(2 * C::MAXB + 2 * j + c) * C::BLOCK]; }
+    __device__ __forceinline__ void drag(int j, float wx, float wy) {
+        dw(j, 0) = wx;
+        dw(j, 1) = wy;
+    }
+    __device__ __forceinline__ void bone(int j, float dx, float dy) {
+        bd(j, 0) = dx;
+        bd(j, 1) = dy;
+    }
+    __device__ __forceinline__ void joint(int j, const Joint& k, float error) {
+        jux[j] = k.ux;
+        juy[j] = k.uy;
+        jvx[j] = k.vx;
+        jvy[j] = k.vy;
+        jerr[j] = error;
+    }
+    __device__ __forceinline__ void ground(int i, float x, float y) {
+        gx[i] = x;
+        gy[i] = y;
+    }
+    __device__ __forceinline__ void damp(int j, float lambda, bool) { dlam[j] = lambda; }
+};
+
+// The adjoints carried in registers: of the energy stores and of the
+// step's demands.
+template <class C>
+struct Grad {
+    float as[C::MAXM], ad[C::MAXM];
+};
+
+// The lane's columns beyond the forward ones: node adjoints (position x, y
+// and velocity x, y) in LDS; each muscle's last reset time (negative: none)
+// and the gradient sums in device memory: muscle k's six at 6k .. 6k+5 in
+// gradient order (amplitude, inverse period, phase, Hill factor, force cap,
+// inverse capacity), then node i's friction at 6 MAXM + i.
+template <class C>
+struct ACols {
+    float4* a;
+    float* rst;
+    float* gs;
+    __device__ __forceinline__ float4& at(uint32_t i) const { return a[i * C::BLOCK]; }
+    __device__ __forceinline__ float& r(uint32_t k) const { return rst[k * C::BLOCK]; }
+    __device__ __forceinline__ float* sum(int q) const { return gs + q * C::BLOCK; }
+    __device__ __forceinline__ void add(int q, float v) const { atomicAddNoRet(sum(q), v); }
+    __device__ __forceinline__ void add_muscle(int k, int f, float v) const { add(6 * k + f, v); }
+    __device__ __forceinline__ void add_fric(int i, float v) const { add(6 * C::MAXM + i, v); }
+};
+
+// Items N-1 down to START that are below the wave's `bound`, unrolled.
+template <int I, int START, class F>
+__device__ __forceinline__ void down_from(uint32_t bound, F& f) {
+    if constexpr (I >= START) {
+        if (static_cast<uint32_t>(I) < bound) { f(std::integral_constant<int, I>()); }
+        down_from<I - 1, START>(bound, f);
+    }
+}
+template <int N, int START, class F>
+__device__ __forceinline__ void items_down(uint32_t bound, F&& f) {
+    down_from<N - 1, START>(bound, f);
+}
+
+// Marks a tape value as unknown to the compiler where the reverse reads it,
+// so that the reverse recomputes what it derives from it instead of keeping
+// the forward pass's copies alive across the whole substep.
+__device__ __forceinline__ float fresh(float x) {
+    asm volatile("" : "+v"(x));
+    return x;
+}
+
+// Marks every constant of the body as changed, at the top of each substep:
+// what the substep derives from them is then computed where it is used, not
+// hoisted out of the step loops and kept in registers for the whole trial.
+__device__ __forceinline__ void opq(float& x) { asm volatile("" : "+v"(x)); }
+__device__ __forceinline__ void opq(uint32_t& x) { asm volatile("" : "+v"(x)); }
+
+template <class C>
+__device__ __forceinline__ void refresh(Body<C>& b) {
+    all_items<C, C::MAXN>([&](auto ic) {
+        const int i = ic;
+        opq(b.nm[i]);
+        opq(b.nw[i]);
+        opq(b.nr[i]);
+        opq(b.nf[i]);
+    });
+    all_items<C, C::MAXB>([&](auto jc) {
+        const int j = jc;
+        opq(b.btopo[j]);
+        opq(b.blen[j]);
+        opq(b.bw0[j]);
+        opq(b.bk[j]);
+        opq(b.blim[j]);
+        opq(b.jlo[j]);
+        opq(b.jhi[j]);
+        opq(b.jwf[j]);
+    });
+    all_items<C, C::MAXM>([&](auto kc) {
+        const int k = kc;
+        opq(b.mnodes[k]);
+        opq(b.maa[k]);
+        opq(b.mab[k]);
+        opq(b.mhill[k]);
+        opq(b.mcap[k]);
+        opq(b.minvcap[k]);
+        if constexpr (C::tendon) {
+            opq(b.mtk[k]);
+            opq(b.mslack[k]);
+        }
+    });
+    opq(b.inv_mass);
+    opq(b.amp);
+    opq(b.qphase);
+}
+
+// Reciprocal and square root for derivative-only arithmetic (about 1 ulp);
+// every value a branch depends on uses the forward pass's own expressions.
+__device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
+// d(slope)/dx of the rolling terrain (shaders/creature.cu:174-186); the
+// other ground shapes have piecewise-constant slopes.
+template <class C>
+__device__ __forceinline__ float ground_curvature(float x, float amp, float qphase, uint32_t flags) {
+    if (!has<C, EVO_TERRAIN>(flags)) { return 0.0f; }
+    const float u0 = fracf(__builtin_fmaf(x, 1.0f / 1.1f, qphase));
+    const float w0 = u0 * (1.0f - u0);
+    const float u1 = fracf(__builtin_fmaf(x, 1.0f / 0.43f, 0.3f) + qphase);
+    const float w1 = u1 * (1.0f - u1);
+    const float c0 = (1.0f - 2.0f * u0) * (1.0f - 2.0f * u0) - 2.0f * w0;
+    const float c1 = (1.0f - 2.0f * u1) * (1.0f - 2.0f * u1) - 2.0f * w1;
+    return amp * (0.65f * 32.0f * c0 * (1.0f / (1.1f * 1.1f)) + 0.35f * 32.0f * c1 * (1.0f / (0.43f * 0.43f)));
+}
+
+// d(ice_at)/dx (shaders/creature.cu:230-235).
+__device__ __forceinline__ float ice_slope(float x) {
+    const float wv = fracf(x * (1.0f / 6.0f));
+    const float t = fabsf(wv - 0.5f) * 2.0f;
+    const float raw = (0.7f - t) * 2.5f;
+    const float s = clampf(raw, 0.0f, 1.0f);
+    const float ds = (!(0.0f > raw) && !(1.0f < raw)) ? -2.5f * 2.0f * (wv < 0.5f ? -1.0f : 1.0f) * (1.0f / 6.0f)
+                                                     : 0.0f;
+    return 6.0f * s * (1.0f - s) * ds;
+}
+
+// d(rhythm)/d(its phase argument) (shaders/creature.cu:153-159); frac has
+// slope 1 and cos(a) has slope -sin(a) = -cos(|a - pi/2|) on [0, pi].
+__device__ __forceinline__ float rhythm_slope(float t, float inv_period, float phase, float offset, float duty,
+                                              float inv_duty, float inv_complement) {
+    const float ph = fracf(__builtin_fmaf(t, inv_period, phase) + offset);
+    const float pi = 3.14159265359f;
+    if (ph < duty) {
+        const float a = pi * ph * inv_duty;
+        return 0.5f * cos_0_pi(fabsf(a - 1.57079632679f)) * pi * inv_duty;
+    }
+    const float a = pi * (ph - duty) * inv_complement;
+    return -0.5f * cos_0_pi(fabsf(a - 1.57079632679f)) * pi * inv_complement;
+}
+
+// The adjoints of a joint's vectors u (parent bone) and v (bone) from those
+// of its gradient vectors gt = (-vy, vx)/|v|^2, gu = (-uy, ux)/|u|^2 and of
+// its angle (the turn from u to v).
+struct UV {
+    float dux, duy, dvx, dvy;
+};
+__device__ __forceinline__ UV geo_back(float ux, float uy, float vx, float vy, float bgtx, float bgty, float bgux,
+                                       float bguy, float bangle) {
+    const float uu = fmaxf(__builtin_fmaf(ux, ux, uy * uy), 1e-12f);
+    const float vv = fmaxf(__builtin_fmaf(vx, vx, vy * vy), 1e-12f);
+    const float ivv = rcp(vv), iuu = rcp(uu);
+    const float i2v = ivv * ivv, i2u = iuu * iuu;
+    UV r;
+    r.dvx = bgtx * (2.0f * vx * vy * i2v) + bgty * (ivv - 2.0f * vx * vx * i2v) - bangle * vy * ivv;
+    r.dvy = bgtx * (2.0f * vy * vy * i2v - ivv) - bgty * (2.0f * vx * vy * i2v) + bangle * vx * ivv;
+    r.dux = bgux * (2.0f * ux * uy * i2u) + bguy * (iuu - 2.0f * ux * ux * i2u) + bangle * uy * iuu;
+    r.duy = bgux * (2.0f * uy * uy * i2u - iuu) - bguy * (2.0f * ux * uy * i2u) - bangle * ux * iuu;
+    return r;
+}
+
+// The reverse of a joint push x += w lambda g on tip, far and hub with
+// lambda = num / wgt: from the adjoints (at, af, ah) of the pushed values,
+// the adjoint of num and those of the gradient vectors gt and gf (gh =
+// -(gt + gf) folded in; gf before the at-head sign).
+struct PushBack {
+    float bnum, bgtx, bgty, bgfx, bgfy;
+};
+__device__ __forceinline__ PushBack push_back(const Joint& k, float lambda, float wgt, float atx, float aty, float afx,
+                                              float afy, float ahx, float ahy) {
+    const float st = k.wt * lambda, sf = k.wf * lambda, sh = k.wh * lambda;
+    const float bst = atx * k.gtx + aty * k.gty;
+    const float bsf = afx * k.gfx + afy * k.gfy;
+    const float bsh = ahx * k.ghx + ahy * k.ghy;
+    const float bl = k.wt * bst + k.wf * bsf + k.wh * bsh;
+    PushBack r;
+    r.bnum = bl * rcp(wgt);
+    const float bwgt2 = -2.0f * r.bnum * lambda;
+    const float bghx = sh * ahx + k.wh * bwgt2 * k.ghx, bghy = sh * ahy + k.wh * bwgt2 * k.ghy;
+    r.bgtx = st * atx + k.wt * bwgt2 * k.gtx - bghx;
+    r.bgty = st * aty + k.wt * bwgt2 * k.gty - bghy;
+    r.bgfx = sf * afx + k.wf * bwgt2 * k.gfx - bghx;
+    r.bgfy = sf * afy + k.wf * bwgt2 * k.gfy - bghy;
+    return r;
+}
+
+// A joint's geometry from its tape (the forward pass's expressions).
+template <class C>
+__device__ __forceinline__ Joint joint_from(const Body<C>& b, int j, uint32_t topo, float ux, float uy, float vx,
+                                           float vy) {
+    Joint k;
+    k.tip = (topo & kHasJoint) != 0u ? static_cast<uint32_t>(j) + 1u : C::trash;
+    k.hub = topo & 63u;
+    k.far_ = (topo >> 18u) & 63u;
+    const bool at_head = (topo & kAtHead) != 0u;
+    k.wt = b.nw[j + 1];
+    k.wf = b.jwf[j];
+    k.wh = b.bw0[j];
+    k.ux = ux;
+    k.uy = uy;
+    k.vx = vx;
+    k.vy = vy;
+    const float uu = fmaxf(__builtin_fmaf(ux, ux, uy * uy), 1e-12f);
+    const float vv = fmaxf(__builtin_fmaf(vx, vx, vy * vy), 1e-12f);
+    const float inv = 1.0f / (uu * vv);
+    const float ivv = uu * inv, iuu = vv * inv;
+    k.gtx = -vy * ivv;
+    k.gty = vx * ivv;
+    const float gux = -uy * iuu, guy = ux * iuu;
+    k.gfx = at_head ? -gux : gux;
+    k.gfy = at_head ? -guy : guy;
+    k.ghx = -(k.gtx + k.gfx);
+    k.ghy = -(k.gty + k.gfy);
+    return k;
+}
+
+// Adds (du, dv) to the position adjoints of the joint's bone vectors: v =
+// tip - hub, u = parent tip - parent pivot.
+template <class C>
+__device__ __forceinline__ void add_uv(const ACols<C>& a, uint32_t topo, uint32_t tip, uint32_t hub, const UV& d) {
+    const uint32_t ptip = (topo >> 6u) & 63u, ppiv = (topo >> 12u) & 63u;
+    float4& rt = a.at(tip);
+    rt.x += d.dvx;
+    rt.y += d.dvy;
+    float4& rh = a.at(hub);
+    rh.x -= d.dvx;
+    rh.y -= d.dvy;
+    float4& rp = a.at(ptip);
+    rp.x += d.dux;
+    rp.y += d.duy;
+    float4& rq = a.at(ppiv);
+    rq.x -= d.dux;
+    rq.y -= d.duy;
+}
+
+// ---------------------------------------------------------------- reverse
+
+// Joint damping (shaders/creature.cu:683-702), last joint first. The node
+// state holds the substep's end: positions final, velocities after every
+// damping push; each push is undone on the way back, which gives the
+// velocities that joint saw.
+template <class C>
+__device__ __forceinline__ void damp_back(const Body<C>& b, const Cols<C>& s, const ACols<C>& a, const Tape<C>& t,
+                                          const Bounds& wb) {
+    items_down<C::MAXB, 1>(wb.b, [&](auto jc) {
+        const int j = jc;
+        uint32_t topo = b.btopo[j];
+        opaque<C>(topo);
+        const Joint k = joint<C>(b, s, j, topo, false);
+        const float wgt = joint_weight(k);
+        const bool on = (topo & kHasJoint) != 0u && wgt > 0.0f;
+        const float lambda = on ? fresh(t.dlam[j]) : 0.0f;
+        float4& pt = s.p(k.tip);
+        float4& pf = s.p(k.far_);
+        float4& ph = s.p(k.hub);
+        float4 vt = pt, vf = pf, vh = ph;
+        const float st = k.wt * lambda, sf = k.wf * lambda, sh = k.wh * lambda;
+        vt.z = __builtin_fmaf(-st, k.gtx, vt.z);
+        vt.w = __builtin_fmaf(-st, k.gty, vt.w);
+        vf.z = __builtin_fmaf(-sf, k.gfx, vf.z);
+        vf.w = __builtin_fmaf(-sf, k.gfy, vf.w);
+        vh.z = __builtin_fmaf(-sh, k.ghx, vh.z);
+        vh.w = __builtin_fmaf(-sh, k.ghy, vh.w);
+        pt = vt;
+        pf = vf;
+        ph = vh;
+        float4& rt = a.at(k.tip);
+        float4& rf = a.at(k.far_);
+        float4& rh = a.at(k.hub);
+        float4 At = rt, Af = rf, Ah = rh;
+        PushBack q = push_back(k, lambda, on ? wgt : 1.0f, At.z, At.w, Af.z, Af.w, Ah.z, Ah.w);
+        const float brate = on ? -C::damp_share * q.bnum : 0.0f;
+        At.z = __builtin_fmaf(brate, k.gtx, At.z);
+        At.w = __builtin_fmaf(brate, k.gty, At.w);
+        Af.z = __builtin_fmaf(brate, k.gfx, Af.z);
+        Af.w = __builtin_fmaf(brate, k.gfy, Af.w);
+        Ah.z = __builtin_fmaf(brate, k.ghx, Ah.z);
+        Ah.w = __builtin_fmaf(brate, k.ghy, Ah.w);
+        const float bgtx = q.bgtx + brate * (vt.z - vh.z), bgty = q.bgty + brate * (vt.w - vh.w);
+        const float bgfx = q.bgfx + brate * (vf.z - vh.z), bgfy = q.bgfy + brate * (vf.w - vh.w);
+        const bool at_head = (topo & kAtHead) != 0u;
+        UV d = geo_back(k.ux, k.uy, k.vx, k.vy, bgtx, bgty, at_head ? -bgfx : bgfx, at_head ? -bgfy : bgfy, 0.0f);
+        if (!on) { d = UV{0.0f, 0.0f, 0.0f, 0.0f}; }
+        rt = At;
+        rf = Af;
+        rh = Ah;
+        add_uv<C>(a, topo, k.tip, k.hub, d);
+    });
+}
+
+// The velocity from positions (shaders/creature.cu:733-740) and the ground
+// (:640-679), node by node: from the adjoint of the end position and
+// velocity, that of the position before the ground; the previous position's
+// share goes to aqx, aqy.
+template <class C>
+__device__ __forceinline__ void ground_back(const Body<C>& b, const Cols<C>& s, const ACols<C>& a, const Tape<C>& t,
+                                            Grad<C>& g, float* aqx, float* aqy, const evo_params& p,
+                                            uint32_t flags, const Bounds& wb) {
+    items<C, C::MAXN, 0, 2>(wb.n, [&](auto ic) {
+        const int i = ic;
+        float4& r = a.at(i);
+        float4 A = r;
+        float ax = __builtin_fmaf(A.z, C::inv_h, A.x), ay = __builtin_fmaf(A.w, C::inv_h, A.y);
+        aqx[i] = -A.z * C::inv_h;
+        aqy[i] = -A.w * C::inv_h;
+        if (has<C, EVO_GROUND>(flags)) {
+            const float x0 = fresh(t.gx[i]), y0 = fresh(t.gy[i]);
+            const float2 prev = s.f(i);
+            const float2 gr = ground<C>(x0, b.amp, b.qphase, p, flags);
+            const float secant = sqrtf(__builtin_fmaf(gr.y, gr.y, 1.0f));
+            const float inv_sec = 1.0f / secant;
+            const float nx = -gr.y * inv_sec, ny = inv_sec;
+            const float dry = __builtin_fmaf(y0 - gr.x, inv_sec, -b.nr[i]);
+            const float depth = -dry;
+            const float mu0 = b.nf[i] * p.friction;
+            float mu = mu0, ice_f = 1.0f;
+            if (has<C, EVO_ICE>(flags)) {
+                ice_f = __builtin_fmaf(-p.patches, ice_at(x0), 1.0f);
+                mu = mu0 * ice_f;
+            }
+            const float x1 = __builtin_fmaf(nx, depth, x0), y1 = __builtin_fmaf(ny, depth, y0);
+            const float tx = ny, ty = -nx;
+            const float slide = __builtin_fmaf(x1 - prev.x, tx, (y1 - prev.y) * ty);
+            const float budget = mu * depth;
+            const bool stick = fabsf(slide) <= budget;
+            const float back = stick ? slide : copysignf(budget, slide);
+            // x2 = x1 - tx back, y2 = y1 - ty back
+            const float bback = -(ax * tx + ay * ty);
+            float btx = -ax * back, bty = -ay * back;
+            const float bslide = stick ? bback : 0.0f;
+            const float bbudget = stick ? 0.0f : (slide < 0.0f ? -bback : bback);
+            const float bmu = bbudget * depth;
+            float bdepth = bbudget * mu;
+            // slide = (x1 - qx) tx + (y1 - qy) ty
+            const float ax1 = ax + bslide * tx, ay1 = ay + bslide * ty;
+            btx += bslide * (x1 - prev.x);
+            bty += bslide * (y1 - prev.y);
+            // tx = ny, ty = -nx; x1 = x0 + nx depth, y1 = y0 + ny depth
+            float bny = btx + ay1 * depth, bnx = -bty + ax1 * depth;
+            bdepth += ax1 * nx + ay1 * ny;
+            // depth = r - (y0 - gh) inv_sec
+            float binv = -bdepth * (y0 - gr.x) - bnx * gr.y + bny;
+            const float bgh = bdepth * inv_sec;
+            // nx = -gs inv_sec, ny = inv_sec, inv_sec = (1 + gs^2)^(-1/2)
+            const float bgs = -bnx * inv_sec - binv * gr.y * inv_sec * inv_sec * inv_sec;
+            float ax0 = ax1, ay0 = ay1 - bdepth * inv_sec;
+            // mu = friction f (1 - patches ice(x0))
+            float dfric = bmu * p.friction * ice_f;
+            if (has<C, EVO_ICE>(flags)) { ax0 += bmu * mu0 * (-p.patches) * ice_slope(x0); }
+            ax0 += bgh * gr.y + bgs * ground_curvature<C>(x0, b.amp, b.qphase, flags);
+            const bool in = depth > 0.0f;
+            aqx[i] -= in ? bslide * tx : 0.0f;
+            aqy[i] -= in ? bslide * ty : 0.0f;
+            a.add_fric(i, in ? dfric : 0.0f);
+            ax = in ? ax0 : ax;
+            ay = in ? ay0 : ay;
+        }
+        A.x = ax;
+        A.y = ay;
+        A.z = 0.0f;
+        A.w = 0.0f;
+        r = A;
+    });
+}
+
+// Joint limits (shaders/creature.cu:618-634), last joint first.
+template <class C>
+__device__ __forceinline__ void joints_back(const Body<C>& b, const ACols<C>& a, const Tape<C>& t,
+                                            const Bounds& wb) {
+    items_down<C::MAXB, 1>(wb.b, [&](auto jc) {
+        const int j = jc;
+        uint32_t topo = b.btopo[j];
+        opaque<C>(topo);
+        const Joint k = joint_from<C>(b, j, topo, fresh(t.jux[j]), fresh(t.juy[j]), fresh(t.jvx[j]), fresh(t.jvy[j]));
+        const float error = fresh(t.jerr[j]);
+        const float wgt = joint_weight(k);
+        const bool on = (topo & kHasJoint) != 0u && error != 0.0f && wgt > 0.0f;
+        const float lambda = on ? -error / wgt : 0.0f;
+        const float4 At = a.at(k.tip), Af = a.at(k.far_), Ah = a.at(k.hub);
+        const PushBack q = push_back(k, lambda, on ? wgt : 1.0f, At.x, At.y, Af.x, Af.y, Ah.x, Ah.y);
+        const bool at_head = (topo & kAtHead) != 0u;
+        // num = -error, error = angle - limit
+        UV d = geo_back(k.ux, k.uy, k.vx, k.vy, q.bgtx, q.bgty, at_head ? -q.bgfx : q.bgfx,
+                        at_head ? -q.bgfy : q.bgfy, -q.bnum);
+        if (!on) { d = UV{0.0f, 0.0f, 0.0f, 0.0f}; }
+        add_uv<C>(a, topo, k.tip, k.hub, d);
+    });
+}
+
+// Bone lengths (shaders/creature.cu:597-613), last bone first.
+template <class C>
+__device__ __forceinline__ void bones_back(const Body<C>& b, const ACols<C>& a, const Tape<C>& t, const Bounds& wb) {
+    items_down<C::MAXB, 0>(wb.b, [&](auto jc) {
+        const int j = jc;
+        uint32_t topo = b.btopo[j];
+        opaque<C>(topo);
+        const uint32_t i0 = topo & 63u;
+        const float dx = t.bd(j, 0), dy = t.bd(j, 1);
+        const float raw = sqrtf(__builtin_fmaf(dx, dx, dy * dy));
+        const float len = fmaxf(raw, 1e-9f);
+        const float w0 = b.bw0[j], w1 = b.nw[j + 1];
+        const float W = w0 + w1;
+        const float il = rcp(len);
+        const float sc = (len - b.blen[j]) * rcp(W) * il;
+        const float s0 = w0 * sc, s1 = w1 * sc;
+        float4& r0 = a.at(i0);
+        float4& r1 = a.at(j + 1);
+        float4 A0 = r0, A1 = r1;
+        const float bs = w0 * (A0.x * dx + A0.y * dy) - w1 * (A1.x * dx + A1.y * dy);
+        float bdx = A0.x * s0 - A1.x * s1, bdy = A0.y * s0 - A1.y * s1;
+        // s = 1/W - L / (W len): d s / d len = L / (W len^2)
+        const float blen = !(1e-9f > raw) ? bs * b.blen[j] * rcp(W) * il * il * il : 0.0f;
+        bdx += blen * dx;
+        bdy += blen * dy;
+        A1.x += bdx;
+        A1.y += bdy;
+        A0.x -= bdx;
+        A0.y -= bdy;
+        r0 = A0;
+        r1 = A1;
+    });
+}
+
+// Integration (shaders/creature.cu:714-728), node by node: the position
+// adjoint moves to the velocity after drag, the force table's adjoint goes
+// to the force-pair column, and the start position's adjoint is completed.
+template <class C>
+__device__ __forceinline__ void integrate_back(const Body<C>& b, const Cols<C>& s, const ACols<C>& a,
+                                               const float* aqx, const float* aqy, const evo_params& p,
+                                               uint32_t flags, const Bounds& wb) {
+    items<C, C::MAXN, 0, 2>(wb.n, [&](auto ic) {
+        const int i = ic;
+        float4& r = a.at(i);
+        float4 A = r;
+        float avx = A.x * C::h, avy = A.y * C::h;
+        if (has<C, EVO_AIR>(flags)) {
+            avx *= p.air_sub;
+            avy *= p.air_sub;
+        }
+        const float hw = C::h * b.nw[i];
+        s.f(i) = make_float2(avx * hw, avy * hw);
+        A.x += aqx[i];
+        A.y += aqy[i];
+        A.z = avx;
+        A.w = avy;
+        r = A;
+    });
+}
+
+// Bone drag in air (shaders/creature.cu:553-593), last bone first.
+template <class C>
+__device__ __forceinline__ void drag_back(const Body<C>& b, const ACols<C>& a, const Tape<C>& t, const Bounds& wb) {
+    items_down<C::MAXB, 0>(wb.b, [&](auto jc) {
+        const int j = jc;
+        uint32_t topo = b.btopo[j];
+        opaque<C>(topo);
+        const uint32_t i0 = topo & 63u;
+        const float wx = t.dw(j, 0), wy = t.dw(j, 1);
+        const float speed = sqrtf(__builtin_fmaf(wx, wx, wy * wy));
+        const float lin = b.bk[j] * speed;
+        const float k_air = fminf(lin, b.blim[j]);
+        float4& r0 = a.at(i0);
+        float4& r1 = a.at(j + 1);
+        float4 A0 = r0, A1 = r1;
+        const float s0 = C::half_h * b.bw0[j], s1 = C::half_h * b.nw[j + 1];
+        const float bdfx = s0 * A0.z + s1 * A1.z, bdfy = s0 * A0.w + s1 * A1.w;
+        float bwx = -bdfx * k_air, bwy = -bdfy * k_air;
+        if (!(b.blim[j] < lin) && speed > 0.0f) {
+            const float bspeed = -(bdfx * wx + bdfy * wy) * b.bk[j] * rcp(speed);
+            bwx += bspeed * wx;
+            bwy += bspeed * wy;
+        }
+        A0.z += 0.5f * bwx;
+        A0.w += 0.5f * bwy;
+        A1.z += 0.5f * bwx;
+        A1.w += 0.5f * bwy;
+        r0 = A0;
+        r1 = A1;
+    });
+}
+
+// Adds c0..c3 to the adjoint rows of nodes a0, a1, b0, b1, any of which may
+// coincide: the four rows are read together, a row named by an earlier slot
+// starts from that slot's new value, and the writes follow in slot order.
+template <class C>
+__device__ __forceinline__ void add4(const ACols<C>& a, uint32_t a0, uint32_t a1, uint32_t b0, uint32_t b1,
+                                     const float4& c0, const float4& c1, const float4& c2, const float4& c3) {
+    float4& r0 = a.at(a0);
+    float4& r1 = a.at(a1);
+    float4& r2 = a.at(b0);
+    float4& r3 = a.at(b1);
+    const float4 v0 = r0, v1 = r1, v2 = r2, v3 = r3;
+    const bool e10 = a1 == a0, e21 = b0 == a1, e20 = b0 == a0, e32 = b1 == b0, e31 = b1 == a1, e30 = b1 == a0;
+    float4 n0, n1, n2, n3;
+    n0.x = v0.x + c0.x;
+    n0.y = v0.y + c0.y;
+    n0.z = v0.z + c0.z;
+    n0.w = v0.w + c0.w;
+    const float4 s1 = e10 ? n0 : v1;
+    n1.x = s1.x + c1.x;
+    n1.y = s1.y + c1.y;
+    n1.z = s1.z + c1.z;
+    n1.w = s1.w + c1.w;
+    const float4 s2 = e21 ? n1 : e20 ? n0 : v2;
+    n2.x = s2.x + c2.x;
+    n2.y = s2.y + c2.y;
+    n2.z = s2.z + c2.z;
+    n2.w = s2.w + c2.w;
+    const float4 s3 = e32 ? n2 : e31 ? n1 : e30 ? n0 : v3;
+    n3.x = s3.x + c3.x;
+    n3.y = s3.y + c3.y;
+    n3.z = s3.z + c3.z;
+    n3.w = s3.w + c3.w;
+    r0 = n0;
+    r1 = n1;
+    r2 = n2;
+    r3 = n3;
+}
+
+// Muscle pulls and energy stores (shaders/creature.cu:455-508), from the
+// substep's start state (node state in LDS, stores in registers) and the
+// force table's adjoint in the force-pair column.
+template <class C>
+__device__ __forceinline__ void muscles_back(const Body<C>& b, const Cols<C>& s, const ACols<C>& a, Grad<C>& g,
+                                             float rec_h, float inv_me, const Bounds& wb) {
+    items<C, C::MAXM, 0, 1>(wb.m, [&](auto kc) {
+        const int k = kc;
+        uint32_t nodes = b.mnodes[k];
+        opaque<C>(nodes);
+        const uint32_t a0 = nodes & 255u, a1 = (nodes >> 8u) & 255u, b0 = (nodes >> 16u) & 255u, b1 = nodes >> 24u;
+        const float aa = b.maa[k], ab = b.mab[k];
+        const float ra = 1.0f - aa, rb = 1.0f - ab;
+        const float4 qa0 = s.p(a0), qa1 = s.p(a1), qb0 = s.p(b0), qb1 = s.p(b1);
+        const float ax = __builtin_fmaf(qa1.x - qa0.x, aa, qa0.x);
+        const float ay = __builtin_fmaf(qa1.y - qa0.y, aa, qa0.y);
+        const float avx = __builtin_fmaf(qa1.z - qa0.z, aa, qa0.z);
+        const float avy = __builtin_fmaf(qa1.w - qa0.w, aa, qa0.w);
+        const float bx = __builtin_fmaf(qb1.x - qb0.x, ab, qb0.x);
+        const float by = __builtin_fmaf(qb1.y - qb0.y, ab, qb0.y);
+        const float bvx = __builtin_fmaf(qb1.z - qb0.z, ab, qb0.z);
+        const float bvy = __builtin_fmaf(qb1.w - qb0.w, ab, qb0.w);
+        const float dx = bx - ax, dy = by - ay;
+        const float raw = sqrtf(__builtin_fmaf(dx, dx, dy * dy));
+        const float len = fmaxf(raw, 1e-6f);
+        const float inv_len = 1.0f / len;
+        const float ex = dx * inv_len, ey = dy * inv_len;
+        const float dvx = bvx - avx, dvy = bvy - avy;
+        const float lengthening = __builtin_fmaf(dvx, ex, dvy * ey);
+        const float store = b.mstore[k];
+        const float drive0 = b.mdemand[k] * store;
+        const float hill = b.mhill[k];
+        const float hraw = __builtin_fmaf(lengthening, hill, 1.0f);
+        const float hc = clampf(hraw, 0.0f, 1.0f);
+        const bool hilled = hill > 0.0f;
+        const float drive = hilled ? drive0 * hc : drive0;
+        const float cap = b.mcap[k];
+        const float q = __builtin_fmaf(lengthening, 0.15f, drive);
+        const float active = clampf(q, -cap, cap);
+        const float mn = fminf(drive, cap), pos = fmaxf(-lengthening, 0.0f);
+        const float work = mn * pos * C::h;
+        const float zin = __builtin_fmaf(rec_h, 1.0f - store, store - work * b.minvcap[k] * inv_me);
+        float pull;
+        bool taut = false;
+        if constexpr (C::tendon) {
+            pull = __builtin_fmaf(b.mtk[k], fmaxf(len - b.mslack[k], 0.0f), active);
+            taut = !(0.0f > len - b.mslack[k]);
+        } else {
+            pull = __builtin_fmaf(0.0f, len, active);
+        }
+        // the force table's adjoint, through each node's share
+        const float2 fa0 = s.f(a0), fa1 = s.f(a1), fb0 = s.f(b0), fb1 = s.f(b1);
+        const float bfx = fa0.x * ra + fa1.x * aa - fb0.x * rb - fb1.x * ab;
+        const float bfy = fa0.y * ra + fa1.y * aa - fb0.y * rb - fb1.y * ab;
+        const float bpull = bfx * ex + bfy * ey;
+        float bex = bfx * pull, bey = bfy * pull;
+        float blen = 0.0f;
+        if constexpr (C::tendon) { blen = taut ? bpull * b.mtk[k] : 0.0f; }
+        // the energy store
+        const float bz = (!(0.0f > zin) && !(1.0f < zin)) ? g.as[k] : 0.0f;
+        float bstore = bz * (1.0f - rec_h);
+        const float bwork = -bz * b.minvcap[k] * inv_me;
+        a.add_muscle(k, 5, -(bz * work * inv_me));
+        // active = clamp(q, -cap, cap)
+        const float bq = (q < -cap || q > cap) ? 0.0f : bpull;
+        float bcap = q < -cap ? -bpull : (q > cap ? bpull : 0.0f);
+        float blength = bq * 0.15f;
+        float bdrive = bq;
+        // work = min(drive, cap) pos h
+        const float bmn = bwork * pos * C::h;
+        const float bpos = bwork * mn * C::h;
+        const bool capped = cap < drive;
+        bdrive += capped ? 0.0f : bmn;
+        bcap += capped ? bmn : 0.0f;
+        blength -= !(0.0f > -lengthening) ? bpos : 0.0f;
+        // drive = drive0 hc (a Hill factor)
+        float bdrive0 = bdrive;
+        if (hilled) {
+            bdrive0 = bdrive * hc;
+            const float bhc = (!(0.0f > hraw) && !(1.0f < hraw)) ? bdrive * drive0 : 0.0f;
+            blength += bhc * hill;
+            a.add_muscle(k, 3, bhc * lengthening);
+        }
+        g.ad[k] += bdrive0 * store;
+        bstore += bdrive0 * b.mdemand[k];
+        g.as[k] = bstore;
+        a.add_muscle(k, 4, bcap);
+        // lengthening = dv . e, e = d / len
+        const float bdvx = blength * ex, bdvy = blength * ey;
+        bex += blength * dvx;
+        bey += blength * dvy;
+        float bdx = bex * inv_len, bdy = bey * inv_len;
+        blen -= (bex * dx + bey * dy) * inv_len * inv_len;
+        if (!(1e-6f > raw)) {
+            bdx += blen * dx * inv_len;
+            bdy += blen * dy * inv_len;
+        }
+        add4<C>(a, a0, a1, b0, b1, make_float4(-bdx * ra, -bdy * ra, -bdvx * ra, -bdvy * ra),
+                make_float4(-bdx * aa, -bdy * aa, -bdvx * aa, -bdvy * aa),
+                make_float4(bdx * rb, bdy * rb, bdvx * rb, bdvy * rb), make_float4(bdx * ab, bdy * ab, bdvx * ab, bdvy * ab));
+    });
+}
+
+// One substep backwards. The node state holds the substep's end state and
+// the force-pair column its start positions (as the taped recomputation left
+// them); `reload` puts the start state back for the muscles.
+template <class C, class Reload>
+__device__ __forceinline__ void substep_back(Body<C>& b, const Cols<C>& s, const ACols<C>& a, const Tape<C>& t,
+                                             Grad<C>& g, const evo_params& p, uint32_t flags, float rec_h,
+                                             const Bounds& wb, Reload&& reload) {
+    float aqx[C::MAXN], aqy[C::MAXN];
+    refresh<C>(b);
+    damp_back<C>(b, s, a, t, wb);
+    ground_back<C>(b, s, a, t, g, aqx, aqy, p, flags, wb);
+    joints_back<C>(b, a, t, wb);
+    bones_back<C>(b, a, t, wb);
+    integrate_back<C>(b, s, a, aqx, aqy, p, flags, wb);
+    drag_back<C>(b, a, t, wb);
+    reload();
+    muscles_back<C>(b, s, a, g, rec_h, p.inv_muscle_energy, wb);
+}
+
+// The step's demands (shaders/creature.cu:442-451) backwards: the demand
+// adjoints go to the rhythm's amplitude, inverse period and phase. After a
+// sensor reset at time r the rhythm runs from (t - r) / period + duty, which
+// no longer depends on the phase. Without sensors in the wave the offsets
+// are zero and no reset has happened.
+template <class C>
+__device__ __forceinline__ void demands_back(const Body<C>& b, const Cols<C>& s, const ACols<C>& a, Grad<C>& g,
+                                             const float* __restrict__ mus, float t, bool sensing,
+                                             const Bounds& wb) {
+    items<C, C::MAXM, 0, 1>(wb.m, [&](auto kc) {
+        const int k = kc;
+        const float bd = g.ad[k];
+        g.ad[k] = 0.0f;
+        if (static_cast<uint32_t>(k) < b.muscles) {
+            const float4* q = reinterpret_cast<const float4*>(mus + k * EVO_MUSCLE_WORDS);
+            const float4 r2 = q[2], r3 = q[3];
+            const float offset = sensing ? s.o(k) : 0.0f;
+            const float reset = sensing ? a.r(k) : -1.0f;
+            const float before = rhythm(t, r2.y, r2.z, offset, r2.w, r3.x, r3.y);
+            const float after = rhythm(t + C::dt, r2.y, r2.z, offset, r2.w, r3.x, r3.y);
+            const float shortening = r2.x * (after - before) * C::rate;
+            if (!(0.0f > shortening * r3.z * 0.25f)) {
+                const float bsh = bd * r3.z * 0.25f;
+                const float bafter = bsh * r2.x * C::rate;
+                const float da = bafter * rhythm_slope(t + C::dt, r2.y, r2.z, offset, r2.w, r3.x, r3.y);
+                const float db = -bafter * rhythm_slope(t, r2.y, r2.z, offset, r2.w, r3.x, r3.y);
+                const float r0 = reset >= 0.0f ? reset : 0.0f;
+                a.add_muscle(k, 0, bsh * (after - before) * C::rate);
+                a.add_muscle(k, 1, da * (t + C::dt - r0) + db * (t - r0));
+                a.add_muscle(k, 2, reset >= 0.0f ? 0.0f : da + db);
+            }
+        }
+    });
+}
+
+// The step's demands (shaders/creature.cu:442-451), the expressions of the
+// simulator's muscle_demands; without sensors in the wave the offsets are
+// zero and are not read.
+template <class C>
+__device__ __forceinline__ void demands(Body<C>& b, const Cols<C>& s, const float* __restrict__ mus, float t,
+                                        bool sensing, const Bounds& wb) {
+    items<C, C::MAXM, 0, 1>(wb.m, [&](auto kc) {
+        const int k = kc;
+        if (static_cast<uint32_t>(k) < b.muscles) {
+            const float4* q = reinterpret_cast<const float4*>(mus + k * EVO_MUSCLE_WORDS);
+            const float4 r2 = q[2], r3 = q[3];
+            const float offset = sensing ? s.o(k) : 0.0f;
+            const float before = rhythm(t, r2.y, r2.z, offset, r2.w, r3.x, r3.y);
+            const float after = rhythm(t + C::dt, r2.y, r2.z, offset, r2.w, r3.x, r3.y);
+            const float shortening = r2.x * (after - before) * C::rate;
+            b.mdemand[k] = fmaxf(shortening * r3.z * 0.25f, 0.0f);
+        }
+    });
+}
+
+// Waits for the lane's outstanding memory operations, so that reads of the
+// gradient sums see every atomic add issued before.
+__device__ __forceinline__ void drain() { __builtin_amdgcn_s_waitcnt(0); }
+
+// Keeps a lane's adjoints inside float range: trajectories that amplify
+// small changes (falls, bounces) can have derivatives far beyond it. When the
+// lane's node or store adjoints pass 2^60, every adjoint and gradient sum of
+// the lane is scaled by 2^-60 and its exponent grows by 60; the gradient is
+// the sums times 2^exponent. Scaling by a power of two is exact.
+template <class C>
+__device__ __forceinline__ void rescale(const Body<C>& b, const ACols<C>& a, Grad<C>& g, int& exponent,
+                                        const Bounds& wb) {
+    float m = 0.0f;
+    items<C, C::MAXN, 0, 2>(wb.n, [&](auto ic) {
+        const int i = ic;
+        const float4 A = a.at(i);
+        const float v = fmaxf(fmaxf(fabsf(A.x), fabsf(A.y)), fmaxf(fabsf(A.z), fabsf(A.w)));
+        m = static_cast<uint32_t>(i) < b.nodes ? fmaxf(m, v) : m;
+    });
+    items<C, C::MAXM, 0, 4>(wb.m, [&](auto kc) {
+        const int k = kc;
+        m = static_cast<uint32_t>(k) < b.muscles ? fmaxf(m, fabsf(g.as[k])) : m;
+    });
+    if (m > 0x1p60f) {
+        const float sc = 0x1p-60f;
+        all_items<C, C::MAXN>([&](auto ic) {
+            const int i = ic;
+            float4& r = a.at(i);
+            float4 A = r;
+            A.x *= sc;
+            A.y *= sc;
+            A.z *= sc;
+            A.w *= sc;
+            r = A;
+        });
+        all_items<C, C::MAXM>([&](auto kc) {
+            const int k = kc;
+            g.as[k] *= sc;
+            g.ad[k] *= sc;
+        });
+        drain();
+        for (int q = 0; q < 6 * C::MAXM + C::MAXN; q++) { *a.sum(q) *= sc; }
+        drain();
+        exponent += 60;
+    }
+}
+
+// ---------------------------------------------------------------- forward
+
+// The centre of mass x (shaders/creature.cu:997-1003), as the simulator sums it.
+template <class C>
+__device__ __forceinline__ float com_x(const Body<C>& b, const Cols<C>& s, const Bounds& wb) {
+    float c = 0.0f;
+    items<C, C::MAXN, 0, 1>(wb.n, [&](auto ic) {
+        const int i = ic;
+        if (static_cast<uint32_t>(i) < b.nodes) { c = __builtin_fmaf(b.nm[i], s.p(i).x, c); }
+    });
+    return c * b.inv_mass;
+}
+
+// Sensor resets at the end of a step (shaders/creature.cu:956-1000): a foot
+// that touches down restarts the rhythm of the muscles it senses.
+template <class C>
+__device__ __forceinline__ void sensors(const Body<C>& b, const Cols<C>& s, const ACols<C>& a,
+                                        const float* __restrict__ mus, const evo_params& p, uint32_t flags,
+                                        uint32_t step, uint32_t& ground_bits, const Bounds& wb) {
+    uint32_t touching = 0u;
+    if (has<C, EVO_GROUND>(flags)) {
+        items<C, C::MAXN, 0, 1>(wb.n, [&](auto ic) {
+            const int i = ic;
+            if (static_cast<uint32_t>(i) < b.nodes) {
+                const float4 q = s.p(i);
+                const float2 g = ground<C>(q.x, b.amp, b.qphase, p, flags);
+                const float floor_y = __builtin_fmaf(b.nr[i], sqrtf(__builtin_fmaf(g.y, g.y, 1.0f)), g.x);
+                if (q.y <= floor_y + 0.002f) { touching |= 1u << i; }
+            }
+        });
+    }
+    const uint32_t down = touching & ~ground_bits;
+    ground_bits = touching;
+    if (down != 0u && step > 0u) {
+        const float t_now = static_cast<float>(step) * C::dt;
+        const float next = t_now + C::dt;
+        for (uint32_t k = 0; k < b.muscles; k++) {
+            const float4* q = reinterpret_cast<const float4*>(mus + k * EVO_MUSCLE_WORDS);
+            const uint32_t sensor = f_bits(q[4].x);
+            if (sensor != EVO_NONE && sensor < 32u && ((down >> sensor) & 1u)) {
+                const float4 r2 = q[2], r3 = q[3];
+                s.o(k) = fracf(r3.w - __builtin_fmaf(next, r2.y, r2.z));
+                a.r(k) = next;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- workspace
+
+// The lane's records in the workspace: checkpoints, substep start states and
+// step rhythm states, each [record][word][lane] with `stride` lanes.
+template <class C>
+struct Work {
+    float* ckpt;
+    float* slots;
+    float* steps;
+    uint64_t stride;
+    uint32_t lane;
+    using W = Words<C::MAXN, C::MAXM>;
+
+    __device__ __forceinline__ void put_nodes(float* base, const Cols<C>& s, const Bounds& wb) const {
+        float4* q = reinterpret_cast<float4*>(base) + lane;
+        items<C, C::MAXN, 0, 4>(wb.n, [&](auto ic) { q[static_cast<int>(ic) * stride] = s.p(static_cast<int>(ic)); });
+    }
+    __device__ __forceinline__ void get_nodes(const float* base, const Cols<C>& s, const Bounds& wb) const {
+        const float4* q = reinterpret_cast<const float4*>(base) + lane;
+        items<C, C::MAXN, 0, 4>(wb.n, [&](auto ic) { s.p(static_cast<int>(ic)) = q[static_cast<int>(ic) * stride]; });
+    }
+    template <class F>
+    __device__ __forceinline__ void muscle_words(float* base, const Bounds& wb, F&& f) const {
+        float* q = base + lane;
+        items<C, C::MAXM, 0, 4>(wb.m, [&](auto kc) { f(static_cast<int>(kc), q[static_cast<int>(kc) * stride]); });
+    }
+    __device__ __forceinline__ float* slot(uint32_t i) const { return slots + i * W::slot * stride; }
+    __device__ __forceinline__ float* check(uint32_t g) const { return ckpt + g * W::ckpt * stride; }
+    __device__ __forceinline__ float* step(uint32_t i) const { return steps + i * W::step * stride; }
+
+    __device__ __forceinline__ void put_slot(uint32_t i, const Body<C>& b, const Cols<C>& s, const Bounds& wb) const {
+        float* base = slot(i);
+        put_nodes(base, s, wb);
+        muscle_words(base + 4u * C::MAXN * stride, wb, [&](int k, float& w) { w = b.mstore[k]; });
+    }
+    __device__ __forceinline__ void get_slot(uint32_t i, Body<C>& b, const Cols<C>& s, const Bounds& wb) const {
+        const float* base = slot(i);
+        get_nodes(base, s, wb);
+        muscle_words(const_cast<float*>(base) + 4u * C::MAXN * stride, wb, [&](int k, float& w) { b.mstore[k] = w; });
+    }
+    __device__ __forceinline__ void put_check(uint32_t g, const Body<C>& b, const Cols<C>& s, const ACols<C>& a,
+                                              uint32_t ground_bits, bool sensing, const Bounds& wb) const {
+        float* base = check(g);
+        put_nodes(base, s, wb);
+        muscle_words(base + 4u * C::MAXN * stride, wb, [&](int k, float& w) { w = b.mstore[k]; });
+        if (sensing) {
+            muscle_words(base + (4u * C::MAXN + C::MAXM) * stride, wb, [&](int k, float& w) { w = s.o(k); });
+            muscle_words(base + (4u * C::MAXN + 2u * C::MAXM) * stride, wb, [&](int k, float& w) { w = a.r(k); });
+            base[(4u * C::MAXN + 3u * C::MAXM) * stride + lane] = bits_f(ground_bits);
+        }
+    }
+    __device__ __forceinline__ void get_check(uint32_t g, Body<C>& b, const Cols<C>& s, const ACols<C>& a,
+                                              uint32_t& ground_bits, bool sensing, const Bounds& wb) const {
+        float* base = check(g);
+        get_nodes(base, s, wb);
+        muscle_words(base + 4u * C::MAXN * stride, wb, [&](int k, float& w) { b.mstore[k] = w; });
+        if (sensing) {
+            muscle_words(base + (4u * C::MAXN + C::MAXM) * stride, wb, [&](int k, float& w) { s.o(k) = w; });
+            muscle_words(base + (4u * C::MAXN + 2u * C::MAXM) * stride, wb, [&](int k, float& w) { a.r(k) = w; });
+            ground_bits = f_bits(base[(4u * C::MAXN + 3u * C::MAXM) * stride + lane]);
+        }
+    }
+    __device__ __forceinline__ void put_step(uint32_t i, const Cols<C>& s, const ACols<C>& a, const Bounds& wb) const {
+        float* base = step(i);
+        muscle_words(base, wb, [&](int k, float& w) { w = s.o(k); });
+        muscle_words(base + C::MAXM * stride, wb, [&](int k, float& w) { w = a.r(k); });
+    }
+    __device__ __forceinline__ void get_step(uint32_t i, const Cols<C>& s, const ACols<C>& a, const Bounds& wb) const {
+        float* base = step(i);
+        muscle_words(base, wb, [&](int k, float& w) { s.o(k) = w; });
+        muscle_words(base + C::MAXM * stride, wb, [&](int k, float& w) { a.r(k) = w; });
+    }
+};
+
+// ---------------------------------------------------------------- kernel
+
+// Wave-wide OR of a lane predicate, as a scalar.
+__device__ __forceinline__ bool wave_any(bool v) { return __ballot(v) != 0ull; }
+
+// One lane per creature p.base + lane (lanes past p.count idle). Writes the
+// gradient (`EVO_ADJ_STRIDE` values per creature, parameter order of the
+// header) and the fitness the recomputation reached.
+template <class C>
+__global__ void __launch_bounds__(64) adjoint_kernel(const uint32_t* __restrict__ records,
+                                                     const float* __restrict__ muscles,
+                                                     const uint4* __restrict__ heads,
+                                                     const evo_result* __restrict__ results, const evo_params p,
+                                                     uint32_t flags, uint32_t seg, float* __restrict__ grads,
+                                                     float* __restrict__ fitness, float* __restrict__ ckpt,
+                                                     float* __restrict__ slots, float* __restrict__ steps_ws,
+                                                     float* __restrict__ rhythm, uint64_t stride) {
+    static_assert(C::BLOCK == 64, "one wave per block");
+    __shared__ float4 s_pv[(C::MAXN + 1) * 64];
+    __shared__ float2 s_fp[(C::MAXN + 1) * 64];
+    __shared__ float4 s_adj[(C::MAXN + 1) * 64];
+    __shared__ float s_tape[Tape<C>::kWords * 64];
+    const uint32_t lane = threadIdx.x;
+    const uint32_t idx = blockIdx.x * 64u + lane;
+    const bool in_launch = idx < p.count;
+    const uint32_t c = p.base + (in_launch ? idx : p.count - 1u);
+    Cols<C> s;
+    s.pv = s_pv + lane;
+    s.fp = s_fp + lane;
+    // the block's device-memory columns: offsets, reset times, gradient sums
+    float* const rcol = rhythm + static_cast<uint64_t>(blockIdx.x) * (Words<C::MAXN, C::MAXM>::columns * 64u) + lane;
+    s.off = rcol;
+    ACols<C> a;
+    a.a = s_adj + lane;
+    a.rst = rcol + C::MAXM * 64;
+    a.gs = rcol + 2 * C::MAXM * 64;
+    const float rec_h = 0.5f * p.muscle_recovery * C::h;
+    Body<C> b;
+    Trial tr;
+    const uint4 h0 = heads[2u * c];
+    const bool fits = (h0.x & 255u) <= static_cast<uint32_t>(C::MAXN) && h0.y <= static_cast<uint32_t>(C::MAXM);
+    begin_creature<C>(b, tr, s, c, records, muscles, heads, p, flags, nullptr);
+    const float* mus = muscles + tr.mus_off;
+    if (!fits) {
+        // a body beyond the class: every index stays inside the columns and
+        // the lane takes no part (the launcher sizes the class from the batch)
+        b.nodes = b.nodes < static_cast<uint32_t>(C::MAXN) ? b.nodes : static_cast<uint32_t>(C::MAXN);
+        b.bones = b.nodes - 1u;
+        b.muscles = 0u;
+        all_items<C, C::MAXM>([&](auto kc) {
+            b.mnodes[static_cast<int>(kc)] = C::trash | (C::trash << 8u) | (C::trash << 16u) | (C::trash << 24u);
+        });
+    }
+    bool sensor = false;
+    for (uint32_t k = 0; k < b.muscles; k++) { sensor = sensor || f_bits(mus[k * EVO_MUSCLE_WORDS + 16]) != EVO_NONE; }
+    all_items<C, C::MAXM>([&](auto kc) {
+        s.o(static_cast<int>(kc)) = 0.0f;
+        a.r(static_cast<int>(kc)) = -1.0f;
+    });
+    const uint32_t code = __float_as_uint(results[c].ground_hi);
+    // the trial's own length, never past the launch's (the workspace's size)
+    uint32_t steps_l = min(code >> 16u, p.steps);
+    if (!in_launch || !fits || (code & 32u) != 0u) { steps_l = 0u; }
+    const uint32_t S = wave_max(steps_l);
+    const bool sensing = wave_any(sensor && steps_l > 0u);
+    const Bounds wb{wave_max(b.nodes), wave_max(b.bones), wave_max(b.muscles)};
+    const Work<C> work{ckpt, slots, steps_ws, stride, idx};
+    uint32_t ground_bits = 0u;
+    float fit = results[c].fitness;
+
+    // forward: the trial with a checkpoint every `seg` steps
+    for (uint32_t st = 0; st < S; st++) {
+        if (st % seg == 0u) { work.put_check(st / seg, b, s, a, ground_bits, sensing, wb); }
+        const float t = static_cast<float>(st) * C::dt;
+        demands<C>(b, s, mus, t, sensing, wb);
+        for (uint32_t sub = 0; sub < C::substeps; sub++) {
+            refresh<C>(b);
+            substep<C>(b, s, p, flags, rec_h, wb);
+        }
+        if (sensing) { sensors<C>(b, s, a, mus, p, flags, st, ground_bits, wb); }
+        if (st + 1u == steps_l) { fit = com_x<C>(b, s, wb); }
+    }
+
+    // the adjoint of the fitness (the centre of mass x at the trial's end)
+    Grad<C> g;
+    all_items<C, C::MAXN>([&](auto ic) {
+        const int i = ic;
+        const float w = static_cast<uint32_t>(i) < b.nodes ? b.nm[i] * b.inv_mass : 0.0f;
+        a.at(i) = make_float4(w, 0.0f, 0.0f, 0.0f);
+    });
+    for (int q = 0; q < 6 * C::MAXM + C::MAXN; q++) { *a.sum(q) = 0.0f; }
+    drain();
+    a.at(C::trash) = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    all_items<C, C::MAXM>([&](auto kc) {
+        const int k = kc;
+        g.as[k] = g.ad[k] = 0.0f;
+    });
+    int exponent = 0;
+
+    const uint32_t nseg = (S + seg - 1u) / seg;
+    for (uint32_t sg = nseg; sg-- > 0u;) {
+        const uint32_t s0 = sg * seg, s1 = min(s0 + seg, S);
+        work.get_check(sg, b, s, a, ground_bits, sensing, wb);
+        for (uint32_t st = s0; st < s1; st++) {
+            if (sensing) { work.put_step(st - s0, s, a, wb); }
+            const float t = static_cast<float>(st) * C::dt;
+            demands<C>(b, s, mus, t, sensing, wb);
+            for (uint32_t sub = 0; sub < C::substeps; sub++) {
+                refresh<C>(b);
+                work.put_slot((st - s0) * C::substeps + sub, b, s, wb);
+                substep<C>(b, s, p, flags, rec_h, wb);
+            }
+            if (sensing) { sensors<C>(b, s, a, mus, p, flags, st, ground_bits, wb); }
+        }
+        for (uint32_t st = s1; st-- > s0;) {
+            const bool live = st < steps_l;
+            if (sensing) { work.get_step(st - s0, s, a, wb); }
+            const float t = static_cast<float>(st) * C::dt;
+            demands<C>(b, s, mus, t, sensing, wb);
+            for (uint32_t sub = C::substeps; sub-- > 0u;) {
+                const uint32_t slot = (st - s0) * C::substeps + sub;
+                refresh<C>(b);
+                work.get_slot(slot, b, s, wb);
+                Tape<C> tape;
+                tape.e = s_tape + lane;
+                substep<C, Tape<C>>(b, s, p, flags, rec_h, wb, &tape);
+                if (live) {
+                    substep_back<C>(b, s, a, tape, g, p, flags, rec_h, wb,
+                                    [&]() { work.get_slot(slot, b, s, wb); });
+                }
+            }
+            if (live) {
+                demands_back<C>(b, s, a, g, mus, t, sensing, wb);
+                rescale<C>(b, a, g, exponent, wb);
+            }
+        }
+    }
+
+    if (!in_launch) { return; }
+    drain();
+    fitness[idx] = fit;
+    float* out = grads + static_cast<uint64_t>(idx) * EVO_ADJ_STRIDE;
+    for (uint32_t q = 0; q < EVO_ADJ_STRIDE; q++) { out[q] = 0.0f; }
+    if (steps_l == 0u) { return; }
+    for (uint32_t q = 0; q < 6u * b.muscles; q++) { out[q] = ldexpf(*a.sum(static_cast<int>(q)), exponent); }
+    for (uint32_t i = 0; i < b.nodes; i++) {
+        out[6u * b.muscles + i] = ldexpf(*a.sum(6 * C::MAXM + static_cast<int>(i)), exponent);
+    }
+}
+
+}  // namespace adj
+}  // namespace dev
+}  // namespace evo

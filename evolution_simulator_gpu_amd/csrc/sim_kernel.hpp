@@ -280,10 +280,19 @@ struct Trial {
 
 // ---------------------------------------------------------------- joints
 
+// What the adjoint's recomputation of a substep keeps of each stage
+// (the inputs of each bone's drag and projection, each joint's geometry and
+// push, each node's position before the ground). The simulator passes none,
+// and its stage functions then compile to the plain stages.
+struct NoTape {};
+template <class T>
+constexpr bool kTaped = !std::is_same<T, NoTape>::value;
+
 // shaders/creature.cu:320-356
 struct Joint {
     uint32_t tip, far_, hub;
     float gtx, gty, gfx, gfy, ghx, ghy, angle, lo, hi, wt, wf, wh;
+    float ux, uy, vx, vy;  // the parent bone's and the bone's vectors
 };
 
 template <class C>
@@ -303,6 +312,10 @@ __device__ __forceinline__ Joint joint(const Body<C>& b, const Cols<C>& s, int j
     const float4 pt = s.p(ptip), pp = s.p(ppiv), tt = s.p(k.tip), hh = s.p(k.hub);
     const float ux = pt.x - pp.x, uy = pt.y - pp.y;
     const float vx = tt.x - hh.x, vy = tt.y - hh.y;
+    k.ux = ux;
+    k.uy = uy;
+    k.vx = vx;
+    k.vy = vy;
     const float uu = fmaxf(__builtin_fmaf(ux, ux, uy * uy), 1e-12f);
     const float vv = fmaxf(__builtin_fmaf(vx, vx, vy * vy), 1e-12f);
     k.angle = 0.0f;
@@ -487,9 +500,9 @@ __device__ __forceinline__ float2 node_force(const Body<C>& b, int i, const floa
 }
 
 // shaders/creature.cu:553-593
-template <class C>
+template <class C, class T = NoTape>
 __device__ __forceinline__ void bone_drag(const Body<C>& b, const Cols<C>& s, const evo_params& p, uint32_t flags,
-                                          const Bounds& wb) {
+                                          const Bounds& wb, T* tape = nullptr) {
     items<C, C::MAXB, 0, 2>(wb.b, [&](auto jc) {
         const int j = jc;
         uint32_t topo = b.btopo[j];
@@ -499,6 +512,7 @@ __device__ __forceinline__ void bone_drag(const Body<C>& b, const Cols<C>& s, co
         float4& q1r = s.p(j + 1);
         float4 q0 = q0r, q1 = q1r;
         const float wx = 0.5f * (q0.z + q1.z), wy = 0.5f * (q0.w + q1.w);
+        if constexpr (kTaped<T>) { tape->drag(j, wx, wy); }
         const float speed = sqrtf(__builtin_fmaf(wx, wx, wy * wy));
         const float limit = b.blim[j];
         const float k_air = fminf(b.bk[j] * speed, limit);
@@ -528,8 +542,9 @@ __device__ __forceinline__ void bone_drag(const Body<C>& b, const Cols<C>& s, co
 }
 
 // shaders/creature.cu:597-613
-template <class C>
-__device__ __forceinline__ void solve_bones(const Body<C>& b, const Cols<C>& s, const Bounds& wb) {
+template <class C, class T = NoTape>
+__device__ __forceinline__ void solve_bones(const Body<C>& b, const Cols<C>& s, const Bounds& wb,
+                                            T* tape = nullptr) {
     items<C, C::MAXB, 0, 1>(wb.b, [&](auto jc) {
         const int j = jc;
         uint32_t topo = b.btopo[j];
@@ -539,6 +554,7 @@ __device__ __forceinline__ void solve_bones(const Body<C>& b, const Cols<C>& s, 
         float4& q1r = s.p(j + 1);
         float4 q0 = q0r, q1 = q1r;
         const float dx = q1.x - q0.x, dy = q1.y - q0.y;
+        if constexpr (kTaped<T>) { tape->bone(j, dx, dy); }
         const float len = fmaxf(sqrtf(__builtin_fmaf(dx, dx, dy * dy)), 1e-9f);
         const float w0 = b.bw0[j], w1 = b.nw[j + 1];
         const float sc = (len - b.blen[j]) / ((w0 + w1) * len);
@@ -553,8 +569,9 @@ __device__ __forceinline__ void solve_bones(const Body<C>& b, const Cols<C>& s, 
 }
 
 // shaders/creature.cu:618-634
-template <class C>
-__device__ __forceinline__ void solve_joints(const Body<C>& b, const Cols<C>& s, const Bounds& wb) {
+template <class C, class T = NoTape>
+__device__ __forceinline__ void solve_joints(const Body<C>& b, const Cols<C>& s, const Bounds& wb,
+                                             T* tape = nullptr) {
     items<C, C::MAXB, 1, 1>(wb.b, [&](auto jc) {
         const int j = jc;
         uint32_t topo = b.btopo[j];
@@ -563,6 +580,7 @@ __device__ __forceinline__ void solve_joints(const Body<C>& b, const Cols<C>& s,
         float error = 0.0f;
         if (k.angle < k.lo) { error = k.angle - k.lo; }
         if (k.angle > k.hi) { error = k.angle - k.hi; }
+        if constexpr (kTaped<T>) { tape->joint(j, k, error); }
         if ((topo & kHasJoint) != 0u && error != 0.0f) {
             const float w = joint_weight(k);
             if (w > 0.0f) { joint_push<C, false>(s, k, -error / w, true); }
@@ -571,8 +589,9 @@ __device__ __forceinline__ void solve_joints(const Body<C>& b, const Cols<C>& s,
 }
 
 // shaders/creature.cu:683-702
-template <class C>
-__device__ __forceinline__ void damp_joints(const Body<C>& b, const Cols<C>& s, const Bounds& wb) {
+template <class C, class T = NoTape>
+__device__ __forceinline__ void damp_joints(const Body<C>& b, const Cols<C>& s, const Bounds& wb,
+                                            T* tape = nullptr) {
     items<C, C::MAXB, 1, 1>(wb.b, [&](auto jc) {
         const int j = jc;
         uint32_t topo = b.btopo[j];
@@ -587,7 +606,9 @@ __device__ __forceinline__ void damp_joints(const Body<C>& b, const Cols<C>& s, 
         rate = __builtin_fmaf(k.ghy, vh.w, rate);
         const float w = joint_weight(k);
         const bool on = (topo & kHasJoint) != 0u && w > 0.0f;
-        joint_push<C, true>(s, k, (-C::damp_share * rate) / w, on);
+        const float lambda = (-C::damp_share * rate) / w;
+        if constexpr (kTaped<T>) { tape->damp(j, lambda, on); }
+        joint_push<C, true>(s, k, lambda, on);
     });
 }
 
@@ -607,12 +628,13 @@ __device__ __forceinline__ uint64_t broken_joints(const Body<C>& b, const Cols<C
 }
 
 // shaders/creature.cu:640-679 plus the velocity of :733-740, for node i.
-template <class C>
+template <class C, class T = NoTape>
 __device__ __forceinline__ void ground_and_velocity(Body<C>& b, const Cols<C>& s, int i, const evo_params& p,
-                                                    uint32_t flags) {
+                                                    uint32_t flags, T* tape = nullptr) {
     float4& qr = s.p(i);
     float4 q = qr;
     const float2 prev = s.f(i);
+    if constexpr (kTaped<T>) { tape->ground(i, q.x, q.y); }
     if (has<C, EVO_GROUND>(flags)) {
         const float2 g = ground<C>(q.x, b.amp, b.qphase, p, flags);
         const float secant = sqrtf(__builtin_fmaf(g.y, g.y, 1.0f));
@@ -671,16 +693,16 @@ __device__ __forceinline__ void integrate(const Body<C>& b, const Cols<C>& s, in
 }
 
 // shaders/creature.cu:705-742 — one substep.
-template <class C>
+template <class C, class T = NoTape>
 __device__ __forceinline__ void substep(Body<C>& b, const Cols<C>& s, const evo_params& p, uint32_t flags,
-                                       float rec_h, const Bounds& wb) {
+                                       float rec_h, const Bounds& wb, T* tape = nullptr) {
     muscle_forces<C>(b, s, rec_h, p.inv_muscle_energy, wb);
-    bone_drag<C>(b, s, p, flags, wb);
+    bone_drag<C, T>(b, s, p, flags, wb, tape);
     items<C, C::MAXN, 0, 2>(wb.n, [&](auto ic) { integrate<C>(b, s, ic, p, flags); });
-    solve_bones<C>(b, s, wb);
-    solve_joints<C>(b, s, wb);
-    items<C, C::MAXN, 0, 2>(wb.n, [&](auto ic) { ground_and_velocity<C>(b, s, ic, p, flags); });
-    damp_joints<C>(b, s, wb);
+    solve_bones<C, T>(b, s, wb, tape);
+    solve_joints<C, T>(b, s, wb, tape);
+    items<C, C::MAXN, 0, 2>(wb.n, [&](auto ic) { ground_and_velocity<C, T>(b, s, ic, p, flags, tape); });
+    damp_joints<C, T>(b, s, wb, tape);
 }
 
 // ---------------------------------------------------------------- per step

@@ -245,8 +245,9 @@ void evo_graph_destroy(evo_graph* graph);
 #define EVO_ADJ_MAX_MUSCLES 24
 #define EVO_ADJ_STRIDE (6u * EVO_ADJ_MAX_MUSCLES + EVO_ADJ_MAX_NODES)
 
-/* Device workspace of an adjoint launch: a checkpoint every `seg` steps and
- * the start state of every substep of one segment, per creature. */
+/* Device workspace of an adjoint launch: per creature a checkpoint every
+ * `seg` steps, the start state of every substep of one segment and the
+ * rhythm state of every step of one segment. */
 uint64_t evo_adjoint_workspace_bytes(uint32_t count, uint32_t steps, uint32_t substeps, uint32_t seg);
 
 /* The backward half of a forward/backward pair (no counterpart in the
@@ -259,7 +260,10 @@ uint64_t evo_adjoint_workspace_bytes(uint32_t count, uint32_t steps, uint32_t su
  * `seg` steps. Worlds: ground, terrain, quake, slope, gaps, hurdles, ice,
  * wind and air (EVO_ERR_INVALID for mud, water and brambles); bodies of at
  * most EVO_ADJ_MAX_NODES nodes and EVO_ADJ_MAX_MUSCLES muscles
- * (run->max_nodes, run->max_muscles). A failed trial has a zero gradient. */
+ * (run->max_nodes, run->max_muscles and run->no_tendons must hold for the
+ * batch: a body beyond them gets a zero gradient); muscles with sensors are
+ * differentiated through their rhythm resets. A failed trial has a zero
+ * gradient. Asynchronous on `stream`. */
 int evo_adjoint_device(const uint32_t* d_records, const float* d_muscles, const uint32_t* d_heads,
                        const evo_result* d_results, const evo_params* p, const evo_run* run, uint32_t seg,
                        float* d_grads, float* d_fitness, void* d_workspace, uint64_t workspace_bytes, void* stream);

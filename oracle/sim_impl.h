@@ -659,7 +659,7 @@ static int FN(run_step)(const FN(World)* w, FN(Body)* b, FN(Trial)* tr, const fl
             const float* m = mus + k * EVO_MUSCLE_WORDS;
             const uint32_t sensor = orc_as_uint(m[16]);
             if (sensor != EVO_NONE && sensor < 32u && ((down >> sensor) & 1u)) {
-                b->moffset[k] = FN(fracr)((R)m[15] - FMA(next, (R)m[9], (R)m[10]));
+                b->moffset[k] = FN(fracr)((R)m[15] - FMA(next, b->minvp[k], b->mphase[k]));
             }
         }
     }

@@ -125,7 +125,11 @@ def test_adjoint_refuses_what_it_cannot_differentiate():
     assert lib.evo_adjoint_device(dummy, dummy, dummy, dummy, C.byref(p), C.byref(run), 60, dummy, dummy, dummy,
                                   16, None) == N.EVO_ERR_INVALID  # workspace too small
     need = lib.evo_adjoint_workspace_bytes(4, p.steps, 8, 60)
-    assert need == (1 + 60 * 8) * (4 * 12 + 24) * 4 * 4
+    # per lane (rounded up to a wave of 64): one checkpoint (node state,
+    # stores, rhythm offsets, reset times, ground bits), 480 substep start
+    # states (node state, stores), 60 step rhythm states and the lane's
+    # columns (offsets, reset times, six gradient sums per muscle, one per node)
+    assert need == (1 * (4 * 12 + 3 * 24 + 1) + 60 * 8 * (4 * 12 + 24) + 60 * 2 * 24 + 8 * 24 + 12) * 64 * 4
 
 
 @pytest.mark.gpu
@@ -170,3 +174,57 @@ def test_gpu_adjoint_segments_do_not_change_the_gradient(gpu):
         torch.cuda.synchronize()
         grads.append(pair.gradient_numpy())
     assert np.array_equal(grads[0], grads[1]) and np.array_equal(grads[1], grads[2])
+    # a rerun is bit for bit the same (each gradient sum takes only its own
+    # lane's atomic adds, in program order)
+    pair.backward()
+    torch.cuda.synchronize()
+    assert np.array_equal(pair.gradient_numpy().view(np.uint32), grads[2].view(np.uint32))
+
+
+@pytest.mark.gpu
+def test_gpu_adjoint_follows_sensor_resets(gpu):
+    """Walkers (tests/fixtures/warp_walker.json) whose feet restart the
+    rhythm of the muscles they sense, with their own ids, and random bodies
+    with sensors: 3 s trials on flat and rough ground, so feet touch down
+    and resets happen. The recomputed fitness equals the forward launch's and
+    the gradient agrees with the FP32 dual oracle's, whose rhythm offsets
+    carry their derivative through each reset."""
+    import torch
+
+    from evolution_simulator_gpu_amd import genome as G
+    from pathlib import Path
+
+    text = (Path(__file__).resolve().parent / "golden" / "warp_walker.json").read_text()
+    rng = np.random.default_rng(8)
+    cs = []
+    for i in range(6):
+        w = G.Creature.from_json(text)
+        w.id = 40 + i
+        cs.append(w)
+    cs += [G.random_creature(rng, int(rng.integers(3, 11)), int(rng.integers(4, 17)), id=200 + i) for i in range(26)]
+    for terrain in (0.0, 0.05):
+        world = E.world(duration=3.0, substeps=8, terrain=terrain)
+        batch = G.pack(cs, world)
+        p, run = E.run_for(batch, world)
+        if run.max_nodes > A.MAX_NODES or run.max_muscles > A.MAX_MUSCLES:
+            pytest.skip("bodies beyond the adjoint's limits")
+        pair = A.Pair(E.DeviceBatch(batch), p, run, seg=25)
+        pair.forward()
+        pair.backward()
+        torch.cuda.synchronize()
+        res = pair.db.results_numpy()
+        assert np.array_equal(pair.fitness.cpu().numpy().view(np.uint32), res["fitness"].view(np.uint32))
+        g = pair.gradient_numpy()
+        sensing = 0
+        for c in range(batch.count):
+            h = batch.heads[c]
+            mus = batch.muscles[int(h[4]):int(h[4]) + int(h[1]) * N.MUSCLE_WORDS].reshape(-1, N.MUSCLE_WORDS)
+            sensing += bool(np.any(mus[:, 16].view(np.uint32) != N.NONE))
+            _, want = oracle.full_gradient(batch, p, run, c, "f32")
+            n = want.size
+            assert np.all(g[c, n:] == 0.0)
+            if (res["ground_hi"][c:c + 1].view(np.uint32)[0] & 32) != 0:
+                continue
+            err = np.abs(g[c, :n] - want).max() / max(np.abs(want).max(), 1e-30)
+            assert err < 1e-3, (terrain, c, err)
+        assert sensing >= 6
