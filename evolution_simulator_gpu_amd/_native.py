@@ -187,7 +187,10 @@ SIGNATURES = {
     "evo_island_worlds": ([_u64, _u32, C.POINTER(WorldConfig), _P], C.c_int),
     "evo_splitmix_finalize": ([_u64], _u64),
     "evo_adjoint_workspace_bytes": ([_u32, _u32, _u32, _u32], _u64),
-    "evo_adjoint_device": ([_P, _P, _P, _P, C.POINTER(Params), C.POINTER(Run), _u32, _P, _P, _P, _u64, _P], C.c_int),
+    "evo_adjoint_forward_device": ([_P, _P, _P, _P, _P, C.POINTER(Params), C.POINTER(Run), _u32, _P, _u64, _P],
+                                   C.c_int),
+    "evo_adjoint_device": ([_P, _P, _P, _P, C.POINTER(Params), C.POINTER(Run), _u32, _P, _P, _P, _u64, _u32, _P],
+                           C.c_int),
     "evo_engine_default": ([C.POINTER(EngineConfig)], None),
     "evo_open": ([C.POINTER(EngineConfig), C.POINTER(C.c_void_p)], C.c_int),
     "evo_close": ([_P], None),

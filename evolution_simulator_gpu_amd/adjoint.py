@@ -28,7 +28,11 @@ DUTY_SPEED = 24.0  # the rhythm's speed limit behind the amplitude cap (m/s)
 
 
 class Pair:
-    """Forward and backward launches over one device-resident batch."""
+    """Forward and backward launches over one device-resident batch. The
+    forward launch (evo_adjoint_forward_device) is the simulator's launch that
+    also writes the checkpoints the backward launch (evo_adjoint_device)
+    recomputes from; `backward(checkpointed=False)` runs after a plain
+    forward launch and recomputes the trial itself."""
 
     def __init__(self, db: DeviceBatch, p: N.Params, run: N.Run, seg: int = 60):
         import torch
@@ -42,19 +46,27 @@ class Pair:
         self.grads = torch.zeros(n * STRIDE, dtype=torch.float32, device=db.device)
         self.fitness = torch.zeros(n, dtype=torch.float32, device=db.device)
 
-    def forward(self, stream=None) -> None:
-        self.db.launch(self.p, self.run, stream)
-
-    def backward(self, stream=None) -> None:
+    def _stream(self, stream):
         import torch
 
-        if stream is None:
-            stream = torch.cuda.current_stream(self.db.records.device)
+        return stream if stream is not None else torch.cuda.current_stream(self.db.records.device)
+
+    def forward(self, stream=None) -> None:
+        s = self._stream(stream)
+        rc = N.lib().evo_adjoint_forward_device(
+            self.db.records.data_ptr(), self.db.muscles.data_ptr(), self.db.heads.data_ptr(),
+            self.db.results.data_ptr(), self.db.counter.data_ptr(), C.byref(self.p), C.byref(self.run), self.seg,
+            self.workspace.data_ptr(), self.workspace.numel(), C.c_void_p(s.cuda_stream))
+        if rc != N.EVO_OK:
+            raise RuntimeError(f"evo_adjoint_forward_device failed: {rc}")
+
+    def backward(self, stream=None, checkpointed: bool = True) -> None:
+        s = self._stream(stream)
         rc = N.lib().evo_adjoint_device(
             self.db.records.data_ptr(), self.db.muscles.data_ptr(), self.db.heads.data_ptr(),
             self.db.results.data_ptr(), C.byref(self.p), C.byref(self.run), self.seg, self.grads.data_ptr(),
-            self.fitness.data_ptr(), self.workspace.data_ptr(), self.workspace.numel(),
-            C.c_void_p(stream.cuda_stream))
+            self.fitness.data_ptr(), self.workspace.data_ptr(), self.workspace.numel(), int(checkpointed),
+            C.c_void_p(s.cuda_stream))
         if rc != N.EVO_OK:
             raise RuntimeError(f"evo_adjoint_device failed: {rc}")
 

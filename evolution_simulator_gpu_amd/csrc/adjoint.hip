@@ -7,6 +7,7 @@
 #include "../../include/evo_engine.h"
 #include "adjoint_kernel.hpp"
 #include "adjoint_launch.hpp"
+#include "launch.hpp"
 
 namespace {
 
@@ -16,6 +17,35 @@ using Large = evo::dev::adj::Words<evo::dev::adj::kLargeNodes, evo::dev::adj::kL
 
 uint32_t segments(uint32_t steps, uint32_t seg) { return (steps + seg - 1u) / seg; }
 uint64_t lanes(uint32_t count) { return (static_cast<uint64_t>(count) + 63u) & ~static_cast<uint64_t>(63u); }
+
+evo::dev::Instance pick_forward(const evo_run& run) {
+    if (run.rate == 60u && run.substeps == 8u) { return evo::dev::pick_ckpt_r60_s8(run); }
+    if (run.rate == 60u && run.substeps == 16u) { return evo::dev::pick_ckpt_r60_s16(run); }
+    if (run.rate == 120u && run.substeps == 8u) { return evo::dev::pick_ckpt_r120_s8(run); }
+    if (run.rate == 120u && run.substeps == 16u) { return evo::dev::pick_ckpt_r120_s16(run); }
+    return evo::dev::Instance{nullptr, nullptr};
+}
+
+// The checks both halves of a pair make.
+int check_pair(const uint32_t* d_records, const float* d_muscles, const uint32_t* d_heads,
+               const evo_result* d_results, const evo_params* p, const evo_run* run, uint32_t seg,
+               const void* d_workspace, uint64_t workspace_bytes) {
+    if (!d_records || !d_muscles || !d_heads || !d_results || !p || !run || !d_workspace) { return EVO_ERR_INVALID; }
+    if (seg == 0u || p->steps == 0u || run->record || (run->flags & ~evo::dev::adj::kFlags) != 0u) {
+        return EVO_ERR_INVALID;
+    }
+    if (run->max_nodes == 0u || run->max_nodes > static_cast<uint32_t>(EVO_ADJ_MAX_NODES) ||
+        run->max_muscles > static_cast<uint32_t>(EVO_ADJ_MAX_MUSCLES)) {
+        return EVO_ERR_INVALID;
+    }
+    if (workspace_bytes < evo_adjoint_workspace_bytes(p->count, p->steps, run->substeps, seg)) {
+        return EVO_ERR_INVALID;
+    }
+    if ((reinterpret_cast<uintptr_t>(d_workspace) & 15u) != 0u || (reinterpret_cast<uintptr_t>(d_heads) & 15u) != 0u) {
+        return EVO_ERR_INVALID;
+    }
+    return EVO_OK;
+}
 
 AdjointLauncher pick(const evo_run& run) {
     if (run.rate == 60u && run.substeps == 8u) { return evo::dev::adj::adjoint_pick_r60_s8(run); }
@@ -33,33 +63,50 @@ uint64_t evo_adjoint_workspace_bytes(uint32_t count, uint32_t steps, uint32_t su
     if (seg == 0u) { return 0u; }
     // the largest class's records bound every class's
     const uint64_t words = static_cast<uint64_t>(segments(steps, seg)) * Large::ckpt +
-                           static_cast<uint64_t>(seg) * substeps * Large::slot +
+                           static_cast<uint64_t>(seg) * substeps * (Large::slot + Large::tape) +
                            static_cast<uint64_t>(seg) * Large::step + Large::columns;
     return words * lanes(count) * sizeof(float);
 }
 
+int evo_adjoint_forward_device(const uint32_t* d_records, const float* d_muscles, const uint32_t* d_heads,
+                               evo_result* d_results, uint32_t* d_counter, const evo_params* p, const evo_run* run,
+                               uint32_t seg, void* d_workspace, uint64_t workspace_bytes, void* stream) {
+    const int rc = check_pair(d_records, d_muscles, d_heads, d_results, p, run, seg, d_workspace, workspace_bytes);
+    if (rc != EVO_OK) { return rc; }
+    if (!d_counter || !pick(*run)) { return EVO_ERR_INVALID; }
+    const evo::dev::Instance in = pick_forward(*run);
+    if (!in.launch) { return EVO_ERR_INVALID; }
+    if (p->count == 0u) { return EVO_OK; }
+    const uint64_t stride = lanes(p->count);
+    const uint32_t segs = segments(p->steps, seg);
+    float* ws = static_cast<float*>(d_workspace);
+    evo::dev::CkptOut ck;
+    if (evo::dev::adj::small_class(*run)) {
+        using namespace evo::dev::adj;
+        const Regions r = regions<kSmallNodes, kSmallMuscles>(ws, stride, segs, seg, run->substeps);
+        ck = evo::dev::CkptOut{r.ckpt, r.cols, stride, seg, kSmallNodes, kSmallMuscles};
+    } else {
+        using namespace evo::dev::adj;
+        const Regions r = regions<kLargeNodes, kLargeMuscles>(ws, stride, segs, seg, run->substeps);
+        ck = evo::dev::CkptOut{r.ckpt, r.cols, stride, seg, kLargeNodes, kLargeMuscles};
+    }
+    const evo::dev::WorldTable none{nullptr, nullptr, nullptr};
+    const hipError_t e = in.launch(d_records, d_muscles, reinterpret_cast<const uint4*>(d_heads), d_results, d_counter,
+                                   *p, run->flags, none, nullptr, 0u, ck, reinterpret_cast<hipStream_t>(stream), 0);
+    if (e == hipErrorOutOfMemory) { return EVO_ERR_OOM; }
+    return e == hipSuccess ? EVO_OK : EVO_ERR_DEVICE;
+}
+
 int evo_adjoint_device(const uint32_t* d_records, const float* d_muscles, const uint32_t* d_heads,
                        const evo_result* d_results, const evo_params* p, const evo_run* run, uint32_t seg,
-                       float* d_grads, float* d_fitness, void* d_workspace, uint64_t workspace_bytes, void* stream) {
-    if (!d_records || !d_muscles || !d_heads || !d_results || !p || !run || !d_grads || !d_fitness || !d_workspace) {
-        return EVO_ERR_INVALID;
-    }
-    if (seg == 0u || p->steps == 0u || run->record || (run->flags & ~evo::dev::adj::kFlags) != 0u) {
-        return EVO_ERR_INVALID;
-    }
-    if (run->max_nodes == 0u || run->max_nodes > static_cast<uint32_t>(EVO_ADJ_MAX_NODES) ||
-        run->max_muscles > static_cast<uint32_t>(EVO_ADJ_MAX_MUSCLES)) {
-        return EVO_ERR_INVALID;
-    }
+                       float* d_grads, float* d_fitness, void* d_workspace, uint64_t workspace_bytes,
+                       uint32_t checkpointed, void* stream) {
+    const int rc = check_pair(d_records, d_muscles, d_heads, d_results, p, run, seg, d_workspace, workspace_bytes);
+    if (rc != EVO_OK) { return rc; }
+    if (!d_grads || !d_fitness) { return EVO_ERR_INVALID; }
     const AdjointLauncher launch = pick(*run);
     if (!launch) { return EVO_ERR_INVALID; }
-    if (workspace_bytes < evo_adjoint_workspace_bytes(p->count, p->steps, run->substeps, seg)) {
-        return EVO_ERR_INVALID;
-    }
     if (p->count == 0u) { return EVO_OK; }
-    if ((reinterpret_cast<uintptr_t>(d_workspace) & 15u) != 0u || (reinterpret_cast<uintptr_t>(d_heads) & 15u) != 0u) {
-        return EVO_ERR_INVALID;
-    }
     AdjointArgs x;
     x.records = d_records;
     x.muscles = d_muscles;
@@ -73,6 +120,7 @@ int evo_adjoint_device(const uint32_t* d_records, const float* d_muscles, const 
     x.fitness = d_fitness;
     x.workspace = static_cast<float*>(d_workspace);
     x.stride = lanes(p->count);
+    x.checkpointed = checkpointed != 0u ? 1u : 0u;
     const hipError_t e = launch(x, reinterpret_cast<hipStream_t>(stream));
     if (e == hipErrorOutOfMemory) { return EVO_ERR_OOM; }
     return e == hipSuccess ? EVO_OK : EVO_ERR_DEVICE;

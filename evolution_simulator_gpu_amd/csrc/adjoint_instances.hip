@@ -26,23 +26,18 @@ namespace {
 
 template <class C>
 hipError_t launch(const AdjointArgs& x, hipStream_t stream) {
-    using W = Words<C::MAXN, C::MAXM>;
-    const uint64_t stride = x.stride;
-    float* ckpt = x.workspace;
-    float* slots = ckpt + static_cast<uint64_t>(x.segments) * W::ckpt * stride;
-    float* steps = slots + static_cast<uint64_t>(x.seg) * C::substeps * W::slot * stride;
-    float* rhythm = steps + static_cast<uint64_t>(x.seg) * W::step * stride;
+    const Regions r = regions<C::MAXN, C::MAXM>(x.workspace, x.stride, x.segments, x.seg, C::substeps);
     const uint32_t blocks = (x.p.count + 63u) / 64u;
     hipLaunchKernelGGL(adjoint_kernel<C>, dim3(blocks), dim3(64), 0, stream, x.records, x.muscles, x.heads,
-                       x.results, x.p, x.flags, x.seg, x.grads, x.fitness, ckpt, slots, steps, rhythm, stride);
+                       x.results, x.p, x.flags, x.seg, x.grads, x.fitness, r.ckpt, r.slots, r.tapes, r.steps, r.cols,
+                       x.stride, x.checkpointed);
     return hipGetLastError();
 }
 
 template <uint32_t WORLD>
 AdjointLauncher pick_world(const evo_run& run) {
     constexpr uint32_t R = EVO_RATE, S = EVO_SUB;
-    if (run.no_tendons && run.max_nodes <= static_cast<uint32_t>(kSmallNodes) &&
-        run.max_muscles <= static_cast<uint32_t>(kSmallMuscles)) {
+    if (small_class(run)) {
         return &launch<Cfg<R, S, WORLD, kSmallNodes, kSmallMuscles, false, 64, true, false>>;
     }
     return &launch<Cfg<R, S, WORLD, kLargeNodes, kLargeMuscles, false, 64, true, true>>;

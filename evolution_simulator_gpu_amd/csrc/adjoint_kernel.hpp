@@ -29,13 +29,15 @@
 // step, live there too.
 //
 // Memory: a checkpoint of the state every `seg` steps; going backwards, a
-// segment is recomputed from its checkpoint with the start state of every
-// substep written to the workspace, and each substep is recomputed once more
-// from there with its tape and reversed. Workspace records are laid out
-// [record][word][lane], so a wave's lanes touch consecutive words.
+// segment is recomputed from its checkpoint with the start state and the
+// tape of every substep written to the workspace, then reversed substep by
+// substep from those records (the end state a substep's reverse starts from
+// is the start state the previous one left in LDS). Workspace records are
+// laid out [record][word][lane], so a wave's lanes touch consecutive words.
 #pragma once
 
 #include "../../include/evo_engine.h"
+#include "adjoint_launch.hpp"
 #include "sim_kernel.hpp"
 
 namespace evo {
@@ -46,23 +48,50 @@ namespace adj {
 constexpr uint32_t kFlags = EVO_GROUND | EVO_TERRAIN | EVO_SLOPE | EVO_GAPS | EVO_HURDLES | EVO_QUAKE | EVO_ICE |
                             EVO_WIND | EVO_AIR;
 
-// Workspace words per lane: a checkpoint (node state, stores, rhythm
-// offsets, reset times, ground bits), a substep's start state (node state,
-// stores), a step's rhythm offsets and reset times.
-template <int MAXN, int MAXM>
-struct Words {
-    static constexpr uint32_t ckpt = 4u * MAXN + 3u * MAXM + 1u;
-    static constexpr uint32_t slot = 4u * MAXN + MAXM;
-    static constexpr uint32_t step = 2u * MAXM;
-    // the lane's columns: rhythm offsets and reset times, then the gradient
-    // sums (six per muscle in gradient order, then the node frictions)
-    static constexpr uint32_t columns = 2u * MAXM + 6u * MAXM + MAXN;
+// Word offsets inside a tape record.
+template <class C>
+struct TapeAt {
+    static constexpr int drag = 0;                       // 2 per bone
+    static constexpr int bone = 2 * C::MAXB;             // 2 per bone
+    static constexpr int joint = 4 * C::MAXB;            // 5 per bone
+    static constexpr int ground = 9 * C::MAXB;           // 2 per node
+    static constexpr int damp = 9 * C::MAXB + 2 * C::MAXN;  // 1 per bone
 };
 
-// One substep's tape: what the reverse needs beyond the substep's start state
-// and its end state. The early half, read last, sits in the lane's LDS
-// column `e`: per bone the mean velocity of its nodes before its drag, then
-// per bone node 1 - node 0 before its projection.
+// Writes one substep's tape to its workspace record (the segment's
+// recomputation); word i of the lane's record is at base[i * 64 + lane].
+template <class C>
+struct TapeOut {
+    float* base;
+    uint32_t lane;
+    using T = TapeAt<C>;
+    __device__ __forceinline__ float& w(int i) const { return base[i * 64 + static_cast<int>(lane)]; }
+    __device__ __forceinline__ void drag(int j, float wx, float wy) const {
+        w(T::drag + 2 * j) = wx;
+        w(T::drag + 2 * j + 1) = wy;
+    }
+    __device__ __forceinline__ void bone(int j, float dx, float dy) const {
+        w(T::bone + 2 * j) = dx;
+        w(T::bone + 2 * j + 1) = dy;
+    }
+    __device__ __forceinline__ void joint(int j, const Joint& k, float error) const {
+        if (j == 0) { return; }
+        w(T::joint + 5 * j) = k.ux;
+        w(T::joint + 5 * j + 1) = k.uy;
+        w(T::joint + 5 * j + 2) = k.vx;
+        w(T::joint + 5 * j + 3) = k.vy;
+        w(T::joint + 5 * j + 4) = error;
+    }
+    __device__ __forceinline__ void ground(int i, float x, float y) const {
+        w(T::ground + 2 * i) = x;
+        w(T::ground + 2 * i + 1) = y;
+    }
+    __device__ __forceinline__ void damp(int j, float lambda, bool) const { w(T::damp + j) = lambda; }
+};
+
+// One substep's tape as the reverse reads it: the early half (drag and bone
+// vectors, read last) copied into the lane's LDS column `e`, the late half
+// (joints, ground, damping, read first) in registers.
 template <class C>
 struct Tape {
     float* e;
@@ -72,26 +101,45 @@ struct Tape {
     static constexpr int kWords = 4 * C::MAXB;
     __device__ __forceinline__ float& dw(int j, int c) const { return e[(2 * j + c) * C::BLOCK]; }
     __device__ __forceinline__ float& bd(int j, int c) const { return e[(2 * C::MAXB + 2 * j + c) * C::BLOCK]; }
-    __device__ __forceinline__ void drag(int j, float wx, float wy) {
-        dw(j, 0) = wx;
-        dw(j, 1) = wy;
+
+    // Reads the wave's record `rec` (word w of the lane's at rec[w * 64 +
+    // lane]) for the wave's bones and nodes: the late half into registers,
+    // the early half straight into the LDS column by direct loads to LDS
+    // (word w of the 64 lanes to the wave-uniform `lds` + 64 w), so that
+    // every read is issued before any is waited for and the early half takes
+    // no registers. Its LDS words are read only after a wait (drain()).
+    __device__ __forceinline__ void load(const float* rec, uint32_t lane, float* lds, const Bounds& wb) {
+        using T = TapeAt<C>;
+        using gfloat = __attribute__((address_space(1))) float;
+        using lfloat = __attribute__((address_space(3))) float;
+        const auto at = [&](int w) { return rec[w * 64 + static_cast<int>(lane)]; };
+        const auto dma = [&](int w) {
+            __builtin_amdgcn_global_load_lds((const gfloat*)(rec + w * 64 + static_cast<int>(lane)),
+                                             (lfloat*)(lds + w * 64), 4, 0, 0);
+        };
+        items<C, C::MAXB, 0, 2>(wb.b, [&](auto jc) {
+            const int j = jc;
+            dma(T::drag + 2 * j);
+            dma(T::drag + 2 * j + 1);
+            dma(T::bone + 2 * j);
+            dma(T::bone + 2 * j + 1);
+        });
+        items<C, C::MAXB, 1, 2>(wb.b, [&](auto jc) {
+            const int j = jc;
+            jux[j] = at(T::joint + 5 * j);
+            juy[j] = at(T::joint + 5 * j + 1);
+            jvx[j] = at(T::joint + 5 * j + 2);
+            jvy[j] = at(T::joint + 5 * j + 3);
+            jerr[j] = at(T::joint + 5 * j + 4);
+            dlam[j] = at(T::damp + j);
+        });
+        items<C, C::MAXN, 0, 2>(wb.n, [&](auto ic) {
+            const int i = ic;
+            gx[i] = at(T::ground + 2 * i);
+            gy[i] = at(T::ground + 2 * i + 1);
+        });
     }
-    __device__ __forceinline__ void bone(int j, float dx, float dy) {
-        bd(j, 0) = dx;
-        bd(j, 1) = dy;
-    }
-    __device__ __forceinline__ void joint(int j, const Joint& k, float error) {
-        jux[j] = k.ux;
-        juy[j] = k.uy;
-        jvx[j] = k.vx;
-        jvy[j] = k.vy;
-        jerr[j] = error;
-    }
-    __device__ __forceinline__ void ground(int i, float x, float y) {
-        gx[i] = x;
-        gy[i] = y;
-    }
-    __device__ __forceinline__ void damp(int j, float lambda, bool) { dlam[j] = lambda; }
+
 };
 
 // The adjoints carried in registers: of the energy stores and of the
@@ -102,19 +150,32 @@ struct Grad {
 };
 
 // The lane's columns beyond the forward ones: node adjoints (position x, y
-// and velocity x, y) in LDS; each muscle's last reset time (negative: none)
-// and the gradient sums in device memory: muscle k's six at 6k .. 6k+5 in
-// gradient order (amplitude, inverse period, phase, Hill factor, force cap,
-// inverse capacity), then node i's friction at 6 MAXM + i.
+// and velocity x, y) in LDS; in device memory, rows of the block's 64 lanes
+// starting at the wave-uniform `cb` (so that every access is a scalar base,
+// the lane's offset and a constant): each muscle's rhythm offset, its last
+// reset time (negative: none), the gradient sums (muscle k's six at 6k ..
+// 6k+5 in gradient order: amplitude, inverse period, phase, Hill factor,
+// force cap, inverse capacity; then node i's friction at 6 MAXM + i) and
+// the muscle constants.
 template <class C>
 struct ACols {
     float4* a;
-    float* rst;
-    float* gs;
+    float* cb;
+    uint32_t lane;
+    static constexpr int kRst = C::MAXM, kSum = 2 * C::MAXM, kMus = 8 * C::MAXM + C::MAXN;
     __device__ __forceinline__ float4& at(uint32_t i) const { return a[i * C::BLOCK]; }
-    __device__ __forceinline__ float& r(uint32_t k) const { return rst[k * C::BLOCK]; }
-    __device__ __forceinline__ float* sum(int q) const { return gs + q * C::BLOCK; }
-    __device__ __forceinline__ void add(int q, float v) const { atomicAddNoRet(sum(q), v); }
+    __device__ __forceinline__ float& row(int w) const { return cb[w * 64 + static_cast<int>(lane)]; }
+    __device__ __forceinline__ float& off(int k) const { return row(k); }
+    __device__ __forceinline__ float& r(int k) const { return row(kRst + k); }
+    __device__ __forceinline__ float* sum(int q) const { return &row(kSum + q); }
+    // The address is formed from a base the compiler cannot see through, at
+    // each add, so the adds' addresses are not all computed ahead and held.
+    __device__ __forceinline__ void add(int q, float v) const {
+        using gfloat = __attribute__((address_space(1))) float;
+        gfloat* base = (gfloat*)(cb + lane);
+        asm volatile("" : "+v"(base));
+        __builtin_amdgcn_global_atomic_fadd_f32(base + (kSum + q) * 64, v);
+    }
     __device__ __forceinline__ void add_muscle(int k, int f, float v) const { add(6 * k + f, v); }
     __device__ __forceinline__ void add_fric(int i, float v) const { add(6 * C::MAXM + i, v); }
 };
@@ -146,7 +207,7 @@ __device__ __forceinline__ float fresh(float x) {
 __device__ __forceinline__ void opq(float& x) { asm volatile("" : "+v"(x)); }
 __device__ __forceinline__ void opq(uint32_t& x) { asm volatile("" : "+v"(x)); }
 
-template <class C>
+template <class C, bool MUSCLES = true>
 __device__ __forceinline__ void refresh(Body<C>& b) {
     all_items<C, C::MAXN>([&](auto ic) {
         const int i = ic;
@@ -166,6 +227,7 @@ __device__ __forceinline__ void refresh(Body<C>& b) {
         opq(b.jhi[j]);
         opq(b.jwf[j]);
     });
+    if constexpr (MUSCLES) {
     all_items<C, C::MAXM>([&](auto kc) {
         const int k = kc;
         opq(b.mnodes[k]);
@@ -179,6 +241,7 @@ __device__ __forceinline__ void refresh(Body<C>& b) {
             opq(b.mslack[k]);
         }
     });
+    }
     opq(b.inv_mass);
     opq(b.amp);
     opq(b.qphase);
@@ -710,21 +773,22 @@ __device__ __forceinline__ void muscles_back(const Body<C>& b, const Cols<C>& s,
 }
 
 // One substep backwards. The node state holds the substep's end state and
-// the force-pair column its start positions (as the taped recomputation left
-// them); `reload` puts the start state back for the muscles.
-template <class C, class Reload>
+// the force-pair column its start positions; `start` puts the start state
+// in the node state once the damping is reversed, and `muscles` loads the
+// muscle constants before the muscles are.
+template <class C, class Start, class Muscles>
 __device__ __forceinline__ void substep_back(Body<C>& b, const Cols<C>& s, const ACols<C>& a, const Tape<C>& t,
                                              Grad<C>& g, const evo_params& p, uint32_t flags, float rec_h,
-                                             const Bounds& wb, Reload&& reload) {
+                                             const Bounds& wb, Start&& start, Muscles&& muscles) {
     float aqx[C::MAXN], aqy[C::MAXN];
-    refresh<C>(b);
     damp_back<C>(b, s, a, t, wb);
+    start();
     ground_back<C>(b, s, a, t, g, aqx, aqy, p, flags, wb);
     joints_back<C>(b, a, t, wb);
     bones_back<C>(b, a, t, wb);
     integrate_back<C>(b, s, a, aqx, aqy, p, flags, wb);
     drag_back<C>(b, a, t, wb);
-    reload();
+    muscles();
     muscles_back<C>(b, s, a, g, rec_h, p.inv_muscle_energy, wb);
 }
 
@@ -744,7 +808,7 @@ __device__ __forceinline__ void demands_back(const Body<C>& b, const Cols<C>& s,
         if (static_cast<uint32_t>(k) < b.muscles) {
             const float4* q = reinterpret_cast<const float4*>(mus + k * EVO_MUSCLE_WORDS);
             const float4 r2 = q[2], r3 = q[3];
-            const float offset = sensing ? s.o(k) : 0.0f;
+            const float offset = sensing ? a.off(k) : 0.0f;
             const float reset = sensing ? a.r(k) : -1.0f;
             const float before = rhythm(t, r2.y, r2.z, offset, r2.w, r3.x, r3.y);
             const float after = rhythm(t + C::dt, r2.y, r2.z, offset, r2.w, r3.x, r3.y);
@@ -767,14 +831,14 @@ __device__ __forceinline__ void demands_back(const Body<C>& b, const Cols<C>& s,
 // simulator's muscle_demands; without sensors in the wave the offsets are
 // zero and are not read.
 template <class C>
-__device__ __forceinline__ void demands(Body<C>& b, const Cols<C>& s, const float* __restrict__ mus, float t,
+__device__ __forceinline__ void demands(Body<C>& b, const ACols<C>& a, const float* __restrict__ mus, float t,
                                         bool sensing, const Bounds& wb) {
     items<C, C::MAXM, 0, 1>(wb.m, [&](auto kc) {
         const int k = kc;
         if (static_cast<uint32_t>(k) < b.muscles) {
             const float4* q = reinterpret_cast<const float4*>(mus + k * EVO_MUSCLE_WORDS);
             const float4 r2 = q[2], r3 = q[3];
-            const float offset = sensing ? s.o(k) : 0.0f;
+            const float offset = sensing ? a.off(k) : 0.0f;
             const float before = rhythm(t, r2.y, r2.z, offset, r2.w, r3.x, r3.y);
             const float after = rhythm(t + C::dt, r2.y, r2.z, offset, r2.w, r3.x, r3.y);
             const float shortening = r2.x * (after - before) * C::rate;
@@ -871,7 +935,7 @@ __device__ __forceinline__ void sensors(const Body<C>& b, const Cols<C>& s, cons
             const uint32_t sensor = f_bits(q[4].x);
             if (sensor != EVO_NONE && sensor < 32u && ((down >> sensor) & 1u)) {
                 const float4 r2 = q[2], r3 = q[3];
-                s.o(k) = fracf(r3.w - __builtin_fmaf(next, r2.y, r2.z));
+                a.off(k) = fracf(r3.w - __builtin_fmaf(next, r2.y, r2.z));
                 a.r(k) = next;
             }
         }
@@ -880,77 +944,127 @@ __device__ __forceinline__ void sensors(const Body<C>& b, const Cols<C>& s, cons
 
 // ---------------------------------------------------------------- workspace
 
-// The lane's records in the workspace: checkpoints, substep start states and
-// step rhythm states, each [record][word][lane] with `stride` lanes.
+// The wave's records in the workspace: checkpoints, substep start states,
+// substep tapes and step rhythm states. Record q of wave v in a region of
+// `words`-word records starts at (q * waves + v) * words * 64: a record is
+// `words` rows of the wave's 64 lanes, so a row's place in it is fixed at
+// compile time and the lanes of a wave touch one contiguous row at a time.
 template <class C>
 struct Work {
     float* ckpt;
     float* slots;
+    float* tapes;
     float* steps;
-    uint64_t stride;
-    uint32_t lane;
+    uint64_t stride;  // lanes in the launch: waves * 64
+    uint32_t wave, lane;
     using W = Words<C::MAXN, C::MAXM>;
 
+    __device__ __forceinline__ float* rec(float* region, uint32_t q, uint32_t words) const {
+        return region + (static_cast<uint64_t>(q) * stride + static_cast<uint64_t>(wave) * 64u) * words;
+    }
+    __device__ __forceinline__ float* slot(uint32_t i) const { return rec(slots, i, W::slot); }
+    __device__ __forceinline__ float* check(uint32_t g) const { return rec(ckpt, g, W::ckpt); }
+    __device__ __forceinline__ float* step(uint32_t i) const { return rec(steps, i, W::step); }
+    __device__ __forceinline__ float* tape(uint32_t i) const { return rec(tapes, i, W::tape); }
+    __device__ __forceinline__ float& word(float* base, int w) const { return base[w * 64 + static_cast<int>(lane)]; }
+    __device__ __forceinline__ float4& row(float* base, int i) const {
+        return reinterpret_cast<float4*>(base)[i * 64 + static_cast<int>(lane)];
+    }
+
     __device__ __forceinline__ void put_nodes(float* base, const Cols<C>& s, const Bounds& wb) const {
-        float4* q = reinterpret_cast<float4*>(base) + lane;
-        items<C, C::MAXN, 0, 4>(wb.n, [&](auto ic) { q[static_cast<int>(ic) * stride] = s.p(static_cast<int>(ic)); });
+        items<C, C::MAXN, 0, 4>(wb.n, [&](auto ic) { row(base, ic) = s.p(static_cast<int>(ic)); });
     }
-    __device__ __forceinline__ void get_nodes(const float* base, const Cols<C>& s, const Bounds& wb) const {
-        const float4* q = reinterpret_cast<const float4*>(base) + lane;
-        items<C, C::MAXN, 0, 4>(wb.n, [&](auto ic) { s.p(static_cast<int>(ic)) = q[static_cast<int>(ic) * stride]; });
+    __device__ __forceinline__ void get_nodes(float* base, const Cols<C>& s, const Bounds& wb) const {
+        items<C, C::MAXN, 0, 4>(wb.n, [&](auto ic) { s.p(static_cast<int>(ic)) = row(base, ic); });
     }
+    // f(k, word) for the wave's muscles, over words `first` + k
     template <class F>
-    __device__ __forceinline__ void muscle_words(float* base, const Bounds& wb, F&& f) const {
-        float* q = base + lane;
-        items<C, C::MAXM, 0, 4>(wb.m, [&](auto kc) { f(static_cast<int>(kc), q[static_cast<int>(kc) * stride]); });
+    __device__ __forceinline__ void muscle_words(float* base, int first, const Bounds& wb, F&& f) const {
+        items<C, C::MAXM, 0, 4>(wb.m, [&](auto kc) { f(static_cast<int>(kc), word(base, first + static_cast<int>(kc))); });
     }
-    __device__ __forceinline__ float* slot(uint32_t i) const { return slots + i * W::slot * stride; }
-    __device__ __forceinline__ float* check(uint32_t g) const { return ckpt + g * W::ckpt * stride; }
-    __device__ __forceinline__ float* step(uint32_t i) const { return steps + i * W::step * stride; }
 
     __device__ __forceinline__ void put_slot(uint32_t i, const Body<C>& b, const Cols<C>& s, const Bounds& wb) const {
         float* base = slot(i);
         put_nodes(base, s, wb);
-        muscle_words(base + 4u * C::MAXN * stride, wb, [&](int k, float& w) { w = b.mstore[k]; });
-    }
-    __device__ __forceinline__ void get_slot(uint32_t i, Body<C>& b, const Cols<C>& s, const Bounds& wb) const {
-        const float* base = slot(i);
-        get_nodes(base, s, wb);
-        muscle_words(const_cast<float*>(base) + 4u * C::MAXN * stride, wb, [&](int k, float& w) { b.mstore[k] = w; });
+        muscle_words(base, 4 * C::MAXN, wb, [&](int k, float& w) { w = b.mstore[k]; });
     }
     __device__ __forceinline__ void put_check(uint32_t g, const Body<C>& b, const Cols<C>& s, const ACols<C>& a,
                                               uint32_t ground_bits, bool sensing, const Bounds& wb) const {
         float* base = check(g);
         put_nodes(base, s, wb);
-        muscle_words(base + 4u * C::MAXN * stride, wb, [&](int k, float& w) { w = b.mstore[k]; });
+        muscle_words(base, 4 * C::MAXN, wb, [&](int k, float& w) { w = b.mstore[k]; });
         if (sensing) {
-            muscle_words(base + (4u * C::MAXN + C::MAXM) * stride, wb, [&](int k, float& w) { w = s.o(k); });
-            muscle_words(base + (4u * C::MAXN + 2u * C::MAXM) * stride, wb, [&](int k, float& w) { w = a.r(k); });
-            base[(4u * C::MAXN + 3u * C::MAXM) * stride + lane] = bits_f(ground_bits);
+            muscle_words(base, 4 * C::MAXN + C::MAXM, wb, [&](int k, float& w) { w = a.off(k); });
+            muscle_words(base, 4 * C::MAXN + 2 * C::MAXM, wb, [&](int k, float& w) { w = a.r(k); });
+            word(base, 4 * C::MAXN + 3 * C::MAXM) = bits_f(ground_bits);
         }
     }
     __device__ __forceinline__ void get_check(uint32_t g, Body<C>& b, const Cols<C>& s, const ACols<C>& a,
                                               uint32_t& ground_bits, bool sensing, const Bounds& wb) const {
         float* base = check(g);
         get_nodes(base, s, wb);
-        muscle_words(base + 4u * C::MAXN * stride, wb, [&](int k, float& w) { b.mstore[k] = w; });
+        muscle_words(base, 4 * C::MAXN, wb, [&](int k, float& w) { b.mstore[k] = w; });
         if (sensing) {
-            muscle_words(base + (4u * C::MAXN + C::MAXM) * stride, wb, [&](int k, float& w) { s.o(k) = w; });
-            muscle_words(base + (4u * C::MAXN + 2u * C::MAXM) * stride, wb, [&](int k, float& w) { a.r(k) = w; });
-            ground_bits = f_bits(base[(4u * C::MAXN + 3u * C::MAXM) * stride + lane]);
+            muscle_words(base, 4 * C::MAXN + C::MAXM, wb, [&](int k, float& w) { a.off(k) = w; });
+            muscle_words(base, 4 * C::MAXN + 2 * C::MAXM, wb, [&](int k, float& w) { a.r(k) = w; });
+            ground_bits = f_bits(word(base, 4 * C::MAXN + 3 * C::MAXM));
         }
     }
     __device__ __forceinline__ void put_step(uint32_t i, const Cols<C>& s, const ACols<C>& a, const Bounds& wb) const {
         float* base = step(i);
-        muscle_words(base, wb, [&](int k, float& w) { w = s.o(k); });
-        muscle_words(base + C::MAXM * stride, wb, [&](int k, float& w) { w = a.r(k); });
+        muscle_words(base, 0, wb, [&](int k, float& w) { w = a.off(k); });
+        muscle_words(base, C::MAXM, wb, [&](int k, float& w) { w = a.r(k); });
     }
     __device__ __forceinline__ void get_step(uint32_t i, const Cols<C>& s, const ACols<C>& a, const Bounds& wb) const {
         float* base = step(i);
-        muscle_words(base, wb, [&](int k, float& w) { s.o(k) = w; });
-        muscle_words(base + C::MAXM * stride, wb, [&](int k, float& w) { a.r(k) = w; });
+        muscle_words(base, 0, wb, [&](int k, float& w) { a.off(k) = w; });
+        muscle_words(base, C::MAXM, wb, [&](int k, float& w) { a.r(k) = w; });
     }
 };
+
+// The lane's muscle constants in device memory (node word, anchors, Hill
+// factor, cap, inverse capacity, tendon stiffness and slack; eight rows a
+// muscle): the reverse holds them in registers only while the muscles run.
+template <class C>
+__device__ __forceinline__ void put_muscles(const ACols<C>& a, const Body<C>& b) {
+    all_items<C, C::MAXM>([&](auto kc) {
+        const int k = kc;
+        const int w = ACols<C>::kMus + 8 * k;
+        a.row(w) = bits_f(b.mnodes[k]);
+        a.row(w + 1) = b.maa[k];
+        a.row(w + 2) = b.mab[k];
+        a.row(w + 3) = b.mhill[k];
+        a.row(w + 4) = b.mcap[k];
+        a.row(w + 5) = b.minvcap[k];
+        if constexpr (C::tendon) {
+            a.row(w + 6) = b.mtk[k];
+            a.row(w + 7) = b.mslack[k];
+        }
+    });
+}
+// Loads the wave's muscles; the base is made opaque first, so the loads stay
+// where they are written.
+template <class C>
+__device__ __forceinline__ void get_muscles(const ACols<C>& a, Body<C>& b, const Bounds& wb) {
+    using gfloat = __attribute__((address_space(1))) float;
+    const gfloat* base = (const gfloat*)a.cb;
+    asm volatile("" : "+s"(base));
+    const auto at = [&](int w) { return base[w * 64 + static_cast<int>(a.lane)]; };
+    items<C, C::MAXM, 0, 2>(wb.m, [&](auto kc) {
+        const int k = kc;
+        const int w = ACols<C>::kMus + 8 * k;
+        b.mnodes[k] = f_bits(at(w));
+        b.maa[k] = at(w + 1);
+        b.mab[k] = at(w + 2);
+        b.mhill[k] = at(w + 3);
+        b.mcap[k] = at(w + 4);
+        b.minvcap[k] = at(w + 5);
+        if constexpr (C::tendon) {
+            b.mtk[k] = at(w + 6);
+            b.mslack[k] = at(w + 7);
+        }
+    });
+}
 
 // ---------------------------------------------------------------- kernel
 
@@ -967,8 +1081,9 @@ __global__ void __launch_bounds__(64) adjoint_kernel(const uint32_t* __restrict_
                                                      const evo_result* __restrict__ results, const evo_params p,
                                                      uint32_t flags, uint32_t seg, float* __restrict__ grads,
                                                      float* __restrict__ fitness, float* __restrict__ ckpt,
-                                                     float* __restrict__ slots, float* __restrict__ steps_ws,
-                                                     float* __restrict__ rhythm, uint64_t stride) {
+                                                     float* __restrict__ slots, float* __restrict__ tapes,
+                                                     float* __restrict__ steps_ws, float* __restrict__ rhythm,
+                                                     uint64_t stride, uint32_t checkpointed) {
     static_assert(C::BLOCK == 64, "one wave per block");
     __shared__ float4 s_pv[(C::MAXN + 1) * 64];
     __shared__ float2 s_fp[(C::MAXN + 1) * 64];
@@ -982,12 +1097,13 @@ __global__ void __launch_bounds__(64) adjoint_kernel(const uint32_t* __restrict_
     s.pv = s_pv + lane;
     s.fp = s_fp + lane;
     // the block's device-memory columns: offsets, reset times, gradient sums
-    float* const rcol = rhythm + static_cast<uint64_t>(blockIdx.x) * (Words<C::MAXN, C::MAXM>::columns * 64u) + lane;
-    s.off = rcol;
+    float* const cb = rhythm + static_cast<uint64_t>(blockIdx.x) * (Words<C::MAXN, C::MAXM>::columns * 64u);
+    // (the simulator's set-up clears the offsets through this pointer)
+    s.off = cb + lane;
     ACols<C> a;
     a.a = s_adj + lane;
-    a.rst = rcol + C::MAXM * 64;
-    a.gs = rcol + 2 * C::MAXM * 64;
+    a.cb = cb;
+    a.lane = lane;
     const float rec_h = 0.5f * p.muscle_recovery * C::h;
     Body<C> b;
     Trial tr;
@@ -1008,7 +1124,7 @@ __global__ void __launch_bounds__(64) adjoint_kernel(const uint32_t* __restrict_
     bool sensor = false;
     for (uint32_t k = 0; k < b.muscles; k++) { sensor = sensor || f_bits(mus[k * EVO_MUSCLE_WORDS + 16]) != EVO_NONE; }
     all_items<C, C::MAXM>([&](auto kc) {
-        s.o(static_cast<int>(kc)) = 0.0f;
+        a.off(static_cast<int>(kc)) = 0.0f;
         a.r(static_cast<int>(kc)) = -1.0f;
     });
     const uint32_t code = __float_as_uint(results[c].ground_hi);
@@ -1018,15 +1134,17 @@ __global__ void __launch_bounds__(64) adjoint_kernel(const uint32_t* __restrict_
     const uint32_t S = wave_max(steps_l);
     const bool sensing = wave_any(sensor && steps_l > 0u);
     const Bounds wb{wave_max(b.nodes), wave_max(b.bones), wave_max(b.muscles)};
-    const Work<C> work{ckpt, slots, steps_ws, stride, idx};
+    const Work<C> work{ckpt, slots, tapes, steps_ws, stride, blockIdx.x, lane};
+    put_muscles<C>(a, b);
     uint32_t ground_bits = 0u;
     float fit = results[c].fitness;
 
-    // forward: the trial with a checkpoint every `seg` steps
-    for (uint32_t st = 0; st < S; st++) {
+    // forward: the trial with a checkpoint every `seg` steps, unless the
+    // pair's forward launch wrote them (then the fitness is its result)
+    for (uint32_t st = 0; st < (checkpointed ? 0u : S); st++) {
         if (st % seg == 0u) { work.put_check(st / seg, b, s, a, ground_bits, sensing, wb); }
         const float t = static_cast<float>(st) * C::dt;
-        demands<C>(b, s, mus, t, sensing, wb);
+        demands<C>(b, a, mus, t, sensing, wb);
         for (uint32_t sub = 0; sub < C::substeps; sub++) {
             refresh<C>(b);
             substep<C>(b, s, p, flags, rec_h, wb);
@@ -1054,33 +1172,68 @@ __global__ void __launch_bounds__(64) adjoint_kernel(const uint32_t* __restrict_
     const uint32_t nseg = (S + seg - 1u) / seg;
     for (uint32_t sg = nseg; sg-- > 0u;) {
         const uint32_t s0 = sg * seg, s1 = min(s0 + seg, S);
+        // the segment forwards from its checkpoint, keeping each substep's
+        // start state and tape; the node state ends as the segment's end
         work.get_check(sg, b, s, a, ground_bits, sensing, wb);
+        get_muscles<C>(a, b, wb);
         for (uint32_t st = s0; st < s1; st++) {
             if (sensing) { work.put_step(st - s0, s, a, wb); }
             const float t = static_cast<float>(st) * C::dt;
-            demands<C>(b, s, mus, t, sensing, wb);
+            demands<C>(b, a, mus, t, sensing, wb);
             for (uint32_t sub = 0; sub < C::substeps; sub++) {
                 refresh<C>(b);
-                work.put_slot((st - s0) * C::substeps + sub, b, s, wb);
-                substep<C>(b, s, p, flags, rec_h, wb);
+                const uint32_t slot = (st - s0) * C::substeps + sub;
+                work.put_slot(slot, b, s, wb);
+                const TapeOut<C> out{work.tape(slot), lane};
+                substep<C, const TapeOut<C>>(b, s, p, flags, rec_h, wb, &out);
             }
             if (sensing) { sensors<C>(b, s, a, mus, p, flags, st, ground_bits, wb); }
         }
+        // then backwards, substep by substep
         for (uint32_t st = s1; st-- > s0;) {
             const bool live = st < steps_l;
             if (sensing) { work.get_step(st - s0, s, a, wb); }
             const float t = static_cast<float>(st) * C::dt;
-            demands<C>(b, s, mus, t, sensing, wb);
+            demands<C>(b, a, mus, t, sensing, wb);
             for (uint32_t sub = C::substeps; sub-- > 0u;) {
                 const uint32_t slot = (st - s0) * C::substeps + sub;
-                refresh<C>(b);
-                work.get_slot(slot, b, s, wb);
+                // (the muscle constants are loaded where the muscles run)
+                refresh<C, false>(b);
+                // the substep's tape and start state, all reads issued
+                // before any is used: the start positions go to the
+                // force-pair column now, the node state once the damping is
+                // reversed
                 Tape<C> tape;
                 tape.e = s_tape + lane;
-                substep<C, Tape<C>>(b, s, p, flags, rec_h, wb, &tape);
+                tape.load(work.tape(slot), lane, s_tape, wb);
+                float* const rec = work.slot(slot);
+                float4 start[C::MAXN];
+                items<C, C::MAXN, 0, 4>(wb.n, [&](auto ic) {
+                    const int i = ic;
+                    start[i] = work.row(rec, i);
+                });
+                work.muscle_words(rec, 4 * C::MAXN, wb, [&](int k, float& w) { b.mstore[k] = w; });
+                items<C, C::MAXN, 0, 4>(wb.n, [&](auto ic) {
+                    const int i = ic;
+                    s.f(i) = make_float2(start[i].x, start[i].y);
+                });
                 if (live) {
-                    substep_back<C>(b, s, a, tape, g, p, flags, rec_h, wb,
-                                    [&]() { work.get_slot(slot, b, s, wb); });
+                    substep_back<C>(
+                        b, s, a, tape, g, p, flags, rec_h, wb,
+                        [&]() {
+                            items<C, C::MAXN, 0, 4>(wb.n, [&](auto ic) {
+                                const int i = ic;
+                                s.p(i) = start[i];
+                            });
+                            // the early tape's direct loads have landed
+                            drain();
+                        },
+                        [&]() { get_muscles<C>(a, b, wb); });
+                } else {
+                    items<C, C::MAXN, 0, 4>(wb.n, [&](auto ic) {
+                        const int i = ic;
+                        s.p(i) = start[i];
+                    });
                 }
             }
             if (live) {

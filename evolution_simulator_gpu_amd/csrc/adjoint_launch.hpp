@@ -17,6 +17,52 @@ namespace adj {
 constexpr int kSmallNodes = 10, kSmallMuscles = 16;
 constexpr int kLargeNodes = EVO_ADJ_MAX_NODES, kLargeMuscles = EVO_ADJ_MAX_MUSCLES;
 
+// The class of a run's bodies: bodies of at most 10 nodes and 16 muscles
+// without tendons run in the small class.
+inline bool small_class(const evo_run& run) {
+    return run.no_tendons && run.max_nodes <= static_cast<uint32_t>(kSmallNodes) &&
+           run.max_muscles <= static_cast<uint32_t>(kSmallMuscles);
+}
+
+// Workspace words per lane: a checkpoint (node state, stores, rhythm
+// offsets, reset times, ground bits), a substep's start state (node state,
+// stores), a step's rhythm offsets and reset times.
+template <int MAXN, int MAXM>
+struct Words {
+    static constexpr uint32_t ckpt = 4u * MAXN + 3u * MAXM + 1u;
+    static constexpr uint32_t slot = 4u * MAXN + MAXM;
+    static constexpr uint32_t step = 2u * MAXM;
+    // a substep's tape: per bone the drag's mean velocity and the bone
+    // vector, per joint its two vectors and error, per node its position
+    // before the ground, per joint the damping multiplier
+    static constexpr uint32_t tape = 10u * (MAXN - 1) + 2u * MAXN;
+    // the lane's columns: rhythm offsets and reset times, the gradient sums
+    // (six per muscle in gradient order, then the node frictions), then the
+    // muscle constants (eight words per muscle)
+    static constexpr uint32_t columns = 2u * MAXM + 6u * MAXM + MAXN + 8u * MAXM;
+};
+
+// The workspace's regions for a class: checkpoints, substep start states,
+// substep tapes, step rhythm states, then the lanes' columns.
+struct Regions {
+    float* ckpt;
+    float* slots;
+    float* tapes;
+    float* steps;
+    float* cols;
+};
+template <int MAXN, int MAXM>
+Regions regions(float* ws, uint64_t stride, uint32_t segments, uint32_t seg, uint32_t substeps) {
+    using W = Words<MAXN, MAXM>;
+    Regions r;
+    r.ckpt = ws;
+    r.slots = r.ckpt + static_cast<uint64_t>(segments) * W::ckpt * stride;
+    r.tapes = r.slots + static_cast<uint64_t>(seg) * substeps * W::slot * stride;
+    r.steps = r.tapes + static_cast<uint64_t>(seg) * substeps * W::tape * stride;
+    r.cols = r.steps + static_cast<uint64_t>(seg) * W::step * stride;
+    return r;
+}
+
 struct AdjointArgs {
     const uint32_t* records;
     const float* muscles;
@@ -28,6 +74,7 @@ struct AdjointArgs {
     float* fitness;
     float* workspace;
     uint64_t stride;  // lanes per workspace word: the launch's count rounded up to whole waves
+    uint32_t checkpointed;  // the checkpoints were written by the pair's forward launch
 };
 
 using AdjointLauncher = hipError_t (*)(const AdjointArgs& x, hipStream_t stream);

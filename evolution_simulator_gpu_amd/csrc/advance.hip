@@ -55,8 +55,9 @@ int launch(const Instance& in, const uint32_t* d_records, const float* d_muscles
     const int cap = grid_cap_from_env();
     if (cap > 0) { grid = grid > 0 ? std::min(grid, cap) : cap; }
     const uint64_t fpc = static_cast<uint64_t>(evo_settle_frames(run.rate)) + p.steps + 1u;
+    const evo::dev::CkptOut none{nullptr, nullptr, 0u, 0u, 0u, 0u};
     return to_code(in.launch(d_records, d_muscles, reinterpret_cast<const uint4*>(d_heads), d_results, d_counter, p,
-                             run.flags, wt, reinterpret_cast<float2*>(d_frames), fpc, stream, grid));
+                             run.flags, wt, reinterpret_cast<float2*>(d_frames), fpc, none, stream, grid));
 }
 
 // Side streams for the concurrent ranges of one generation, created once per

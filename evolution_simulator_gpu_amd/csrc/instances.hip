@@ -47,13 +47,13 @@ int max_resident_blocks() {
 template <class C>
 hipError_t launch(const uint32_t* records, const float* muscles, const uint4* heads, evo_result* results,
                   uint32_t* counter, const evo_params& p, uint32_t flags, const WorldTable& worlds, float2* frames,
-                  uint64_t fpc, hipStream_t stream, int grid) {
+                  uint64_t fpc, const CkptOut& ck, hipStream_t stream, int grid) {
     const int need = static_cast<int>((p.count + C::BLOCK - 1u) / C::BLOCK);
     grid = std::min(need, grid > 0 ? grid : max_resident_blocks<C>());
     grid = std::max(grid, 1);
     hipLaunchKernelGGL(clear_words<0>, dim3(1), dim3(64), 0, stream, counter, 1u);
     hipLaunchKernelGGL(advance_kernel<C>, dim3(grid), dim3(C::BLOCK), 0, stream, records, muscles, heads, results,
-                       counter, p, flags, worlds, frames, fpc);
+                       counter, p, flags, worlds, frames, fpc, ck);
     return hipGetLastError();
 }
 
@@ -84,6 +84,22 @@ Instance pick_size(const evo_run& run) {
 }
 
 }  // namespace
+
+#define EVO_PICK_CKPT EVO_CAT(pick_ckpt_r, EVO_RATE, _s, EVO_SUB)
+
+Instance EVO_PICK_CKPT(const evo_run& run) {
+    constexpr uint32_t R = EVO_RATE, S = EVO_SUB;
+    if (run.record || run.max_nodes == 0u || run.max_nodes > static_cast<uint32_t>(kMidN) ||
+        run.max_muscles > static_cast<uint32_t>(kMidM)) {
+        return Instance{nullptr, nullptr};
+    }
+    if (run.flags == kFlat) {
+        return run.no_tendons ? instance<Cfg<R, S, kFlat, kMidN, kMidM, false, 128, true, false, true>>()
+                              : instance<Cfg<R, S, kFlat, kMidN, kMidM, false, 128, true, true, true>>();
+    }
+    return run.no_tendons ? instance<Cfg<R, S, kAnyWorld, kMidN, kMidM, false, 128, true, false, true>>()
+                          : instance<Cfg<R, S, kAnyWorld, kMidN, kMidM, false, 128, true, true, true>>();
+}
 
 Instance EVO_PICK(const evo_run& run, bool table) {
     if (run.record) {

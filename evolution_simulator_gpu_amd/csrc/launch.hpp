@@ -20,10 +20,23 @@ struct WorldTable {
     const uint16_t* of;
 };
 
+// Where a forward launch that opens a forward/backward pair writes the
+// adjoint's checkpoints (null `ckpt`: none). Launch creature `at` is lane
+// at % 64 of wave at / 64 in the adjoint's records: every `seg` steps its
+// state goes to checkpoint record step / seg (`an` node rows, then energy
+// stores, rhythm offsets and reset times of `am` muscles, then the ground
+// bits), and its sensor reset times are kept in its column of `cols`.
+struct CkptOut {
+    float* ckpt;
+    float* cols;
+    uint64_t stride;  // lanes of the adjoint launch: waves * 64
+    uint32_t seg, an, am;
+};
+
 using Launcher = hipError_t (*)(const uint32_t* records, const float* muscles, const uint4* heads,
                                 evo_result* results, uint32_t* counter, const evo_params& p, uint32_t flags,
                                 const WorldTable& worlds, float2* frames, uint64_t frames_per_creature,
-                                hipStream_t stream, int grid);
+                                const CkptOut& ck, hipStream_t stream, int grid);
 
 // A compiled kernel instance: its launcher (grid = blocks to launch, or 0
 // for as many as fit on the GPU at once) and the number of its blocks that
@@ -40,6 +53,13 @@ Instance pick_r60_s8(const evo_run& run, bool table);
 Instance pick_r60_s16(const evo_run& run, bool table);
 Instance pick_r120_s8(const evo_run& run, bool table);
 Instance pick_r120_s16(const evo_run& run, bool table);
+
+// The checkpoint-writing instance for a run whose bodies fit the unrolled
+// mid class, or {nullptr, nullptr}.
+Instance pick_ckpt_r60_s8(const evo_run& run);
+Instance pick_ckpt_r60_s16(const evo_run& run);
+Instance pick_ckpt_r120_s8(const evo_run& run);
+Instance pick_ckpt_r120_s16(const evo_run& run);
 
 // Body-size classes: the largest node and muscle counts each unrolls to.
 // The last class runs its loops at run time and has no such limit below the

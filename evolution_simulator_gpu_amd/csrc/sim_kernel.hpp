@@ -49,10 +49,12 @@ constexpr uint32_t kAnyWorld = 0xffffffffu;
 constexpr uint32_t kTableWorld = 0xfffffffeu;
 
 template <uint32_t RATE, uint32_t SUB, uint32_t WORLD, int MAXN_, int MAXM_, bool RECORD, int BLOCK_, bool UNROLLED,
-          bool TENDON>
+          bool TENDON, bool CKPT = false>
 struct Cfg {
     static constexpr bool unrolled = UNROLLED;
     static constexpr bool tendon = TENDON;
+    // writes the adjoint's checkpoints (the forward half of a forward/backward pair)
+    static constexpr bool ckpt = CKPT;
     static constexpr int MAXN = MAXN_;
     static constexpr int MAXB = MAXN_ - 1;
     static constexpr int MAXM = MAXM_;
@@ -934,15 +936,54 @@ __device__ __forceinline__ void step_metrics(const Body<C>& b, const Cols<C>& s,
     }
 }
 
+// The adjoint's view of launch creature `at`: lane at % 64 of wave at / 64,
+// row w of its column and of its checkpoint records at w * 64 + lane.
+__device__ __forceinline__ float& ckpt_rst(const CkptOut& ck, uint32_t at, uint32_t k) {
+    const uint64_t cols = 16ull * ck.am + ck.an;
+    return ck.cols[(static_cast<uint64_t>(at >> 6u) * cols + ck.am + k) * 64u + (at & 63u)];
+}
+
+// Checkpoint g of launch creature `at`: node rows, energy stores, rhythm
+// offsets, reset times and ground bits, as the adjoint's own forward pass
+// writes them.
+template <class C>
+__device__ __forceinline__ void write_checkpoint(const Body<C>& b, const Trial& tr, const Cols<C>& s,
+                                                 const CkptOut& ck, uint32_t at, uint32_t g) {
+    const uint64_t words = 4ull * ck.an + 3ull * ck.am + 1ull;
+    const uint32_t l = at & 63u;
+    float* rec = ck.ckpt + (static_cast<uint64_t>(g) * ck.stride + (at & ~63u)) * words;
+    float4* rows = reinterpret_cast<float4*>(rec);
+    // (a record holds `an` nodes and `am` muscles; the launcher picks the
+    // record class from the run, so the body fits unless the run says less)
+    const uint32_t nodes = min(b.nodes, ck.an), mcount = min(b.muscles, ck.am);
+    all_items<C, C::MAXN>([&](auto ic) {
+        const int i = ic;
+        if (static_cast<uint32_t>(i) < nodes) { rows[i * 64 + static_cast<int>(l)] = s.p(i); }
+    });
+    for (uint32_t k = 0; k < mcount; k++) {
+        rec[(4u * ck.an + ck.am + k) * 64u + l] = s.o(k);
+        rec[(4u * ck.an + 2u * ck.am + k) * 64u + l] = ckpt_rst(ck, at, k);
+    }
+    all_items<C, C::MAXM>([&](auto kc) {
+        const int k = kc;
+        if (static_cast<uint32_t>(k) < mcount) { rec[(4u * ck.an + static_cast<uint32_t>(k)) * 64u + l] = b.mstore[k]; }
+    });
+    rec[(4u * ck.an + 3u * ck.am) * 64u + l] = bits_f(tr.ground_bits);
+}
+
 // shaders/creature.cu:956-1248 — one step of the lane's creature.
 template <class C>
 __device__ __forceinline__ void run_step(Body<C>& b, Trial& tr, const Cols<C>& s, const float* __restrict__ muscles,
                                          evo_result* __restrict__ results, const evo_params& p, uint32_t flags,
-                                         float rec_h, float2* __restrict__ frames, const Bounds& wb) {
+                                         float rec_h, float2* __restrict__ frames, const Bounds& wb,
+                                         const CkptOut& ck) {
     const uint32_t step = tr.step;
     const float* mus = muscles + tr.mus_off;
     const uint32_t cflags = tr.flags;
     const float t_now = static_cast<float>(step) * C::dt;
+    if constexpr (C::ckpt) {
+        if (step % ck.seg == 0u) { write_checkpoint<C>(b, tr, s, ck, tr.cidx - p.base, step / ck.seg); }
+    }
     if constexpr (C::record) {
         all_items<C, C::MAXN>([&](auto ic) {
             b.normal_force[static_cast<int>(ic)] = 0.0f;
@@ -978,6 +1019,9 @@ __device__ __forceinline__ void run_step(Body<C>& b, Trial& tr, const Cols<C>& s
             if (sensor != EVO_NONE && sensor < 32u && ((down >> sensor) & 1u)) {
                 const float4 r2 = q[2], r3 = q[3];
                 s.o(k) = fracf(r3.w - __builtin_fmaf(next, r2.y, r2.z));
+                if constexpr (C::ckpt) {
+                    if (k < ck.am) { ckpt_rst(ck, tr.cidx - p.base, k) = next; }
+                }
             }
         }
     }
@@ -1176,7 +1220,7 @@ __global__ void __launch_bounds__(C::BLOCK) advance_kernel(const uint32_t* __res
                                                           uint32_t* __restrict__ counter, const evo_params p,
                                                           uint32_t flags, const WorldTable worlds,
                                                           float2* __restrict__ frames,
-                                                          uint64_t frames_per_creature) {
+                                                          uint64_t frames_per_creature, const CkptOut ck) {
     __shared__ float4 s_pv[(C::MAXN + 1) * C::BLOCK];
     __shared__ float2 s_fp[(C::MAXN + 1) * C::BLOCK];
     __shared__ float s_off[(C::unrolled && C::MAXM > 0 ? C::MAXM : 1) * C::BLOCK];
@@ -1250,6 +1294,9 @@ __global__ void __launch_bounds__(C::BLOCK) advance_kernel(const uint32_t* __res
             } else {
                 begin_creature<C>(b, tr, s, p.base + at, records, muscles, heads, p, flags, my_frames);
             }
+            if constexpr (C::ckpt) {
+                for (uint32_t k = 0; k < ck.am; k++) { ckpt_rst(ck, at, k) = -1.0f; }
+            }
         }
         if (__ballot(tr.live) == 0ull) { break; }
         const Bounds wb{wave_max(tr.live ? b.nodes : 0u), wave_max(tr.live ? b.bones : 0u),
@@ -1262,9 +1309,9 @@ __global__ void __launch_bounds__(C::BLOCK) advance_kernel(const uint32_t* __res
                 opaque<C>(w);
                 const evo_params& pw = worlds.params[w];
                 const float rh = 0.5f * pw.muscle_recovery * C::h;
-                run_step<C>(b, tr, s, muscles, results, pw, worlds.flags[w], rh, my_frames, wb);
+                run_step<C>(b, tr, s, muscles, results, pw, worlds.flags[w], rh, my_frames, wb, ck);
             } else {
-                run_step<C>(b, tr, s, muscles, results, p, flags, rec_h, my_frames, wb);
+                run_step<C>(b, tr, s, muscles, results, p, flags, rec_h, my_frames, wb, ck);
             }
         }
     }

@@ -250,23 +250,38 @@ void evo_graph_destroy(evo_graph* graph);
  * rhythm state of every step of one segment. */
 uint64_t evo_adjoint_workspace_bytes(uint32_t count, uint32_t steps, uint32_t substeps, uint32_t seg);
 
+/* The forward half of a forward/backward pair (no counterpart in the
+ * reference, which has no gradients): evo_advance_device on the batch (the
+ * same results, bit for bit) that also writes to `d_workspace` the
+ * checkpoints every `seg` steps that evo_adjoint_device with `checkpointed`
+ * set reads, so the backward half need not run the trial forward again.
+ * Same arguments and limits as evo_adjoint_device, plus the claim counter of
+ * evo_advance_device. Asynchronous on `stream`. */
+int evo_adjoint_forward_device(const uint32_t* d_records, const float* d_muscles, const uint32_t* d_heads,
+                               evo_result* d_results, uint32_t* d_counter, const evo_params* p, const evo_run* run,
+                               uint32_t seg, void* d_workspace, uint64_t workspace_bytes, void* stream);
+
 /* The backward half of a forward/backward pair (no counterpart in the
  * reference, which has no gradients): given the results of
- * evo_advance_device on the same batch and run, writes for creatures
- * [p->base, p->base + p->count) d(fitness)/d(parameters) to
- * d_grads[(c - base) * EVO_ADJ_STRIDE ...] and the fitness that the
- * recomputed trajectory reaches (equal to the forward result) to
- * d_fitness[c - base]. The trajectory is recomputed from checkpoints every
- * `seg` steps. Worlds: ground, terrain, quake, slope, gaps, hurdles, ice,
- * wind and air (EVO_ERR_INVALID for mud, water and brambles); bodies of at
- * most EVO_ADJ_MAX_NODES nodes and EVO_ADJ_MAX_MUSCLES muscles
- * (run->max_nodes, run->max_muscles and run->no_tendons must hold for the
- * batch: a body beyond them gets a zero gradient); muscles with sensors are
- * differentiated through their rhythm resets. A failed trial has a zero
- * gradient. Asynchronous on `stream`. */
+ * evo_advance_device (or evo_adjoint_forward_device) on the same batch and
+ * run, writes for creatures [p->base, p->base + p->count)
+ * d(fitness)/d(parameters) to d_grads[(c - base) * EVO_ADJ_STRIDE ...] and
+ * the fitness of the trajectory it differentiates (equal to the forward
+ * result) to d_fitness[c - base]. The trajectory is recomputed from
+ * checkpoints every `seg` steps: with `checkpointed` set, those that
+ * evo_adjoint_forward_device wrote to the same workspace with the same batch,
+ * run and seg; otherwise the backward half runs the trial forward itself
+ * first. Worlds: ground, terrain, quake, slope, gaps, hurdles, ice, wind and
+ * air (EVO_ERR_INVALID for mud, water and brambles); bodies of at most
+ * EVO_ADJ_MAX_NODES nodes and EVO_ADJ_MAX_MUSCLES muscles (run->max_nodes,
+ * run->max_muscles and run->no_tendons must hold for the batch: a body beyond
+ * them gets a zero gradient); muscles with sensors are differentiated through
+ * their rhythm resets. A failed trial has a zero gradient; components beyond
+ * float range come back as +-inf. Asynchronous on `stream`. */
 int evo_adjoint_device(const uint32_t* d_records, const float* d_muscles, const uint32_t* d_heads,
                        const evo_result* d_results, const evo_params* p, const evo_run* run, uint32_t seg,
-                       float* d_grads, float* d_fitness, void* d_workspace, uint64_t workspace_bytes, void* stream);
+                       float* d_grads, float* d_fitness, void* d_workspace, uint64_t workspace_bytes,
+                       uint32_t checkpointed, void* stream);
 
 /* Archive metrics and niche cells of creatures [base, base + count) from
  * their results, on the device: scheduler::to_metrics and trial_steps

@@ -116,20 +116,28 @@ def test_adjoint_refuses_what_it_cannot_differentiate():
     lib = N.lib()
     dummy = C.c_void_p(16)
     assert lib.evo_adjoint_device(dummy, dummy, dummy, dummy, C.byref(p), C.byref(run), 60, dummy, dummy, dummy,
-                                  1 << 40, None) == N.EVO_ERR_INVALID
+                                  1 << 40, 0, None) == N.EVO_ERR_INVALID
+    assert lib.evo_adjoint_forward_device(dummy, dummy, dummy, dummy, dummy, C.byref(p), C.byref(run), 60, dummy,
+                                          1 << 40, None) == N.EVO_ERR_INVALID
     p, run = E.run_for(batch, E.world(duration=1.0, substeps=8))
     run.max_nodes = 13
     assert lib.evo_adjoint_device(dummy, dummy, dummy, dummy, C.byref(p), C.byref(run), 60, dummy, dummy, dummy,
-                                  1 << 40, None) == N.EVO_ERR_INVALID
+                                  1 << 40, 1, None) == N.EVO_ERR_INVALID
+    assert lib.evo_adjoint_forward_device(dummy, dummy, dummy, dummy, dummy, C.byref(p), C.byref(run), 60, dummy,
+                                          1 << 40, None) == N.EVO_ERR_INVALID
     run.max_nodes = 10
     assert lib.evo_adjoint_device(dummy, dummy, dummy, dummy, C.byref(p), C.byref(run), 60, dummy, dummy, dummy,
-                                  16, None) == N.EVO_ERR_INVALID  # workspace too small
+                                  16, 0, None) == N.EVO_ERR_INVALID  # workspace too small
+    assert lib.evo_adjoint_forward_device(dummy, dummy, dummy, dummy, None, C.byref(p), C.byref(run), 60, dummy,
+                                          1 << 40, None) == N.EVO_ERR_INVALID  # no claim counter
     need = lib.evo_adjoint_workspace_bytes(4, p.steps, 8, 60)
     # per lane (rounded up to a wave of 64): one checkpoint (node state,
     # stores, rhythm offsets, reset times, ground bits), 480 substep start
-    # states (node state, stores), 60 step rhythm states and the lane's
-    # columns (offsets, reset times, six gradient sums per muscle, one per node)
-    assert need == (1 * (4 * 12 + 3 * 24 + 1) + 60 * 8 * (4 * 12 + 24) + 60 * 2 * 24 + 8 * 24 + 12) * 64 * 4
+    # states (node state, stores) and tapes (10 words per bone, 2 per node),
+    # 60 step rhythm states and the lane's columns (offsets, reset times, six
+    # gradient sums per muscle and one per node, eight constants per muscle)
+    assert need == (1 * (4 * 12 + 3 * 24 + 1) + 60 * 8 * (4 * 12 + 24 + 10 * 11 + 2 * 12) + 60 * 2 * 24
+                    + 8 * 24 + 12 + 8 * 24) * 64 * 4
 
 
 @pytest.mark.gpu
@@ -179,6 +187,47 @@ def test_gpu_adjoint_segments_do_not_change_the_gradient(gpu):
     pair.backward()
     torch.cuda.synchronize()
     assert np.array_equal(pair.gradient_numpy().view(np.uint32), grads[2].view(np.uint32))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("terrain", [0.0, 0.05])
+def test_gpu_forward_checkpoints_equal_the_backward_own(gpu, terrain):
+    """The pair's forward launch gives the plain launch's results bit for bit
+    and writes the checkpoints the backward half would have written itself:
+    the gradients from either are identical, with walkers whose sensors reset
+    their rhythms among the bodies."""
+    import torch
+
+    from evolution_simulator_gpu_amd import genome as G
+    from pathlib import Path
+
+    text = (Path(__file__).resolve().parent / "golden" / "warp_walker.json").read_text()
+    rng = np.random.default_rng(21)
+    cs = []
+    for i in range(4):
+        w = G.Creature.from_json(text)
+        w.id = 70 + i
+        cs.append(w)
+    cs += [G.random_creature(rng, int(rng.integers(3, 11)), int(rng.integers(4, 17)), id=300 + i) for i in range(60)]
+    world = E.world(duration=2.0, substeps=8, terrain=terrain)
+    batch = G.pack(cs, world)
+    p, run = E.run_for(batch, world)
+    plain = E.simulate(batch, world)
+    pair = A.Pair(E.DeviceBatch(batch), p, run, seg=16)
+    pair.forward()
+    torch.cuda.synchronize()
+    res = pair.db.results_numpy()
+    for key in plain.dtype.names:
+        assert np.array_equal(res[key].view(np.uint32), plain[key].view(np.uint32)), key
+    pair.backward(checkpointed=True)
+    torch.cuda.synchronize()
+    with_ckpt = pair.gradient_numpy().copy()
+    fit = pair.fitness.cpu().numpy().copy()
+    pair.backward(checkpointed=False)
+    torch.cuda.synchronize()
+    assert np.array_equal(pair.gradient_numpy().view(np.uint32), with_ckpt.view(np.uint32))
+    assert np.array_equal(pair.fitness.cpu().numpy().view(np.uint32), fit.view(np.uint32))
+    assert np.abs(with_ckpt[np.isfinite(with_ckpt)]).max() > 0
 
 
 @pytest.mark.gpu
