@@ -52,9 +52,12 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--replays", type=int, default=64)
-    ap.add_argument("--workload", choices=["forward", "islands", "adjoint"], default="forward",
+    ap.add_argument("--workload", choices=["forward", "islands", "adjoint", "full"], default="forward",
                     help="forward: 3M creatures, flat ground (default); islands: 3M creatures of 3-32 nodes over "
-                         "100 island worlds, batched vs per-shape launches")
+                         "100 island worlds, batched vs per-shape launches; adjoint: the forward/backward pair "
+                         "on 65,536 creatures; full: 3M creatures of 3-32 nodes in a hard world, the pair on "
+                         "the 262,144 fittest, gradient-directed mutation and 1,024 replays")
+    ap.add_argument("--elites", type=int, default=262_144)
     ap.add_argument("--islands", type=int, default=100)
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only run 3 device-to-device copies of this many MiB (the scale of the HBM byte counters)")
@@ -296,6 +299,161 @@ def adjoint(args, torch):
     print(json.dumps(line))
 
 
+def hard_world(duration: float):
+    """The full-stack workload's world in the reference's level tables
+    (src/environment.rs:120-160): boulders = terrain bumps of 25 cm, 2 g,
+    wet grip (GRIP[3] = 0.6), ice bands every 6 m removing 80 % of the grip
+    (PATCHES[2]), gaps (GAPS[1] = 0.35 m, the reference's spacing 2 + 4 w)
+    and low hurdles (HURDLES[1] = 0.08 m, every 3 m), tremor (QUAKE[1] = 0.05,
+    the reference's static per-creature bump phase and scale)."""
+    import evolution_simulator_gpu_amd as E
+
+    return E.world(duration=duration, substeps=8, rate=60, terrain=0.25, gravity=19.6, ground_friction=0.6,
+                   patches=0.8, gaps=0.35, hurdles=0.08, quake=0.05)
+
+
+def full(args, torch):
+    """One generation of the full stack: 3,000,000 creatures of 3-32 nodes
+    in the hard world (concurrent per-class persistent kernels from a HIP
+    graph), then the forward/backward pair on the fittest `--elites` that
+    the adjoint's body classes hold (<= 12 nodes, <= 24 muscles), the
+    gradient-directed mutation of those elites, and replay capture of the
+    `--replays` fittest."""
+    import oracle  # the checker of the sampled gradients
+
+    import evolution_simulator_gpu_amd as E
+    from evolution_simulator_gpu_amd import adjoint as A
+    from evolution_simulator_gpu_amd.generation import IslandGeneration
+
+    replays = args.replays if args.replays != 64 else 1024
+    t0 = time.perf_counter()
+    world = hard_world(args.duration)
+    spec = E.synth_spec(4, args.creatures, world=world)
+    batch = E.synth(spec)
+    batch.validate()
+    note(f"generated {batch.count} creatures in {time.perf_counter() - t0:.1f} s")
+    gen = IslandGeneration(batch, [world], np.zeros(batch.count, np.uint32))
+    tuning = gen.tune(seconds=2.0, progress=note)
+    gen.capture()
+    p, run = E.run_for(batch, world)
+    nodes, muscles = batch.nodes(), batch.muscle_counts()
+    for _ in range(args.warmup):
+        gen.step()
+    torch.cuda.synchronize()
+
+    ev = lambda: torch.cuda.Event(enable_timing=True)  # noqa: E731
+    rows = []
+    for _ in range(args.steps):
+        t_gen = time.perf_counter()
+        e = [ev() for _ in range(2)]
+        e[0].record(gen.stream)
+        gen.step()
+        e[1].record(gen.stream)
+        torch.cuda.synchronize()
+        forward_ms = e[0].elapsed_time(e[1])
+        note(f"forward {forward_ms:.0f} ms")
+        # the elites: the fittest creatures of the generation
+        t1 = time.perf_counter()
+        fit = gen.db.results.view(-1, 19)[:, 0]
+        fit = torch.nan_to_num(fit, nan=float("-inf"))
+        elites = torch.topk(fit, min(args.elites, batch.count)).indices.cpu().numpy()
+        fits = (nodes[elites] <= A.MAX_NODES) & (muscles[elites] <= A.MAX_MUSCLES)
+        chosen = np.sort(elites[fits])
+        eb = batch.subset(chosen)
+        select_s = time.perf_counter() - t1
+        note(f"{chosen.size} of {elites.size} elites in the adjoint's classes, gathered in {select_s:.1f} s")
+        # the forward/backward pair on them
+        pe, re = E.run_for(eb, world)
+        pair = A.Pair(E.DeviceBatch(eb), pe, re, seg=60)
+        torch.cuda.synchronize()
+        e = [ev() for _ in range(3)]
+        e[0].record()
+        pair.forward()
+        e[1].record()
+        pair.backward()
+        e[2].record()
+        torch.cuda.synchronize()
+        pair_fwd_ms, pair_bwd_ms = e[0].elapsed_time(e[1]), e[1].elapsed_time(e[2])
+        note(f"pair on {eb.count} elites: forward {pair_fwd_ms:.0f} ms, backward {pair_bwd_ms:.0f} ms")
+        grads = pair.gradient_numpy()
+        # gradient-directed mutation of every elite with a finite gradient
+        t2 = time.perf_counter()
+        finite = np.isfinite(grads).all(axis=1)
+        mutated = A.ascend_batch(eb, np.where(finite[:, None], grads, 0.0), rate=0.02)
+        mutate_s = time.perf_counter() - t2
+        # replay capture of the fittest
+        e = [ev() for _ in range(2)]
+        e[0].record()
+        pick, frames, replay_res = gen.db.capture_top(replays, p, run)
+        e[1].record()
+        torch.cuda.synchronize()
+        replay_ms = e[0].elapsed_time(e[1])
+        wall = time.perf_counter() - t_gen
+        note(f"mutation {mutate_s:.1f} s, replays {replay_ms:.0f} ms, generation {wall:.1f} s")
+        rows.append((wall, forward_ms, select_s, pair_fwd_ms, pair_bwd_ms, mutate_s, replay_ms))
+    res = gen.results()
+    substeps = int((res["ground_hi"].view(np.uint32) >> 16).astype(np.int64).sum()) * world.substeps
+    pres = pair.db.results_numpy()
+    elite_substeps = int((pres["ground_hi"].view(np.uint32) >> 16).astype(np.int64).sum()) * world.substeps
+    same = bool(np.array_equal(pres["fitness"].view(np.uint32), res["fitness"][chosen].view(np.uint32)))
+    # sampled check of the elites' gradients against the FP32 dual oracle
+    rng = np.random.default_rng(12)
+    ok = np.nonzero(finite & ((pres["ground_hi"].view(np.uint32) & 32) == 0))[0]
+    sample = rng.choice(ok, min(64, ok.size), replace=False)
+    dirs = [rng.normal(size=oracle.param_count(eb, int(c))) for c in sample]
+    _, d32 = oracle.directional(eb, pe, re, sample, dirs, "f32", threads=16)
+    mine = np.array([grads[c, :len(d)] @ d for c, d in zip(sample, dirs)])
+    rel32 = np.abs(mine - d32) / np.maximum(np.abs(d32), 1e-12)
+    wall, forward_ms, select_s, pf, pb, mutate_s, replay_ms = (float(np.mean(x)) for x in zip(*rows))
+    line = {
+        "metric": "creature-substeps/s at 3M creatures, 20 s trials, full stack (hard world, adjoint on elites, "
+                  "replays)",
+        "value": (substeps + elite_substeps * 2) / wall,
+        "unit": "creature-substeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "full stack: 3,000,000 creatures from seed 5, 3-32 nodes (geometric p = 0.25), bones + "
+                        "muscles <= 64, hard world (terrain 0.25 m, 2 g, grip 0.6, ice 0.8, gaps 0.35 m, "
+                        "hurdles 0.08 m, tremor 0.05), 20 s trials at 60 Hz x 8; forward/backward pair "
+                        "(checkpoints every 60 steps) on the fittest elites within the adjoint's classes; "
+                        "gradient-directed mutation; replay capture",
+            "creatures": batch.count,
+            "substeps_per_generation": substeps,
+            "forward_ms": forward_ms,
+            "forward_creature_substeps_per_s": substeps / (forward_ms / 1e3),
+            "elites": int(elites.size),
+            "elites_with_gradients": int(chosen.size),
+            "elite_coverage": float(chosen.size / max(elites.size, 1)),
+            "elite_select_gather_s": select_s,
+            "pair_forward_ms": pf,
+            "pair_backward_ms": pb,
+            "pair_over_plain_forward_of_elites": (pf + pb) / pf,
+            "elite_fitness_bit_exact": same,
+            "elite_gradients_beyond_float_range": int((~finite).sum()),
+            "mutation_s": mutate_s,
+            "mutated_records_valid": bool(mutated.validate() is None),
+            "replays": replays,
+            "replay_ms": replay_ms,
+            "elite_gradients_vs_fp32_dual_oracle": {"sample": int(sample.size),
+                                                    "median_rel": finite_or_none(np.median(rel32)),
+                                                    "within_1e-3": float((rel32 < 1e-3).mean())},
+            "tuning": tuning,
+            "value_counts": "forward substeps + 2 x the elites' substeps (forward and backward halves)",
+        },
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -318,6 +476,8 @@ def main():
         return islands(args, torch)
     if args.workload == "adjoint":
         return adjoint(args, torch)
+    if args.workload == "full":
+        return full(args, torch)
 
     import evolution_simulator_gpu_amd as E
     from evolution_simulator_gpu_amd import _native as N

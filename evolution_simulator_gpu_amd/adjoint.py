@@ -199,6 +199,82 @@ def ascend(batch: Batch, creatures, grads: np.ndarray, rate: float = 0.02) -> Ba
 GENES = ("period", "phase", "contracted", "extended", "strength", "friction")
 
 
+def _rows(batch: Batch):
+    """Per muscle of the batch: its creature, its index within the creature
+    and its record row; per node: its creature, its index and the word of
+    its friction."""
+    h = batch.heads
+    m = h[:, 1].astype(np.int64)
+    n = (h[:, 0] & 255).astype(np.int64)
+    mc = np.repeat(np.arange(batch.count), m)
+    mk = np.arange(int(m.sum())) - np.repeat(np.cumsum(m) - m, m)
+    mrow = np.repeat(h[:, 4].astype(np.int64) // N.MUSCLE_WORDS, m) + mk
+    nc = np.repeat(np.arange(batch.count), n)
+    ni = np.arange(int(n.sum())) - np.repeat(np.cumsum(n) - n, n)
+    nword = np.repeat(h[:, 3].astype(np.int64), n) + ni * N.NODE_WORDS + 2
+    return mc, mk, mrow, nc, ni, nword
+
+
+def ascend_batch(batch: Batch, grads: np.ndarray, rate: float = 0.02) -> Batch:
+    """`ascend` for every creature of `batch` at once (grads[c] is creature
+    c's gradient row): the same arithmetic, element for element, over the
+    whole muscle and node arrays, so a generation's elites mutate in one
+    pass. Returns a new batch."""
+    f32 = np.float32
+    mc, mk, mrow, nc, ni, nword = _rows(batch)
+    mus = batch.muscles.reshape(-1, N.MUSCLE_WORDS)[mrow].astype(np.float64)
+    fr = batch.records[nword].view(np.float32).astype(np.float64)
+    grads = np.asarray(grads, np.float64)
+    g = [grads[mc, 6 * mk + f] for f in range(6)]
+    gf = grads[nc, 6 * batch.heads[nc, 1].astype(np.int64) + ni]
+    period, duty = 1.0 / mus[:, 9], mus[:, 11]
+    extended = 1.0 / (8.0 * mus[:, 3])
+    genes = {"period": period, "phase": mus[:, 10].copy(), "extended": extended,
+             "contracted": extended - mus[:, 8], "strength": mus[:, 4] / 200.0, "friction": fr}
+    # the gene gradient (gene_gradient, element for element)
+    longest, strength = genes["extended"], genes["strength"]
+    cap_amp = 2.0 * DUTY_SPEED * period * np.minimum(duty, 1.0 - duty) / np.pi
+    capped = (longest - genes["contracted"]) > cap_amp
+    g_amp, g_invp, g_phase, g_hill, g_cap, g_invcap = g
+    gg = {
+        "period": g_invp * (-1.0 / period ** 2) + np.where(capped, g_amp * cap_amp / period, 0.0),
+        "phase": g_phase,
+        "contracted": np.where(capped, 0.0, -g_amp),
+        "extended": np.where(capped, 0.0, g_amp) + np.where(longest > 0.05, g_hill * (-1.0 / (8.0 * longest ** 2)), 0.0),
+        "strength": g_cap * 200.0 + g_invcap * (-1.0 / (120.0 * strength ** 2)),
+        "friction": gf,
+    }
+    for key in GENES:
+        owner = nc if key == "friction" else mc
+        scale = np.maximum(np.abs(genes[key]), 1e-3)
+        d = gg[key] * scale
+        norm = np.zeros(batch.count)
+        np.maximum.at(norm, owner, np.abs(d))
+        nz = norm[owner] > 0.0
+        step = np.where(nz, rate * scale * d / np.where(nz, norm[owner], 1.0), 0.0)
+        genes[key] = np.where(nz, genes[key] + step, genes[key])
+    genes["period"] = np.clip(genes["period"], 0.5, 2.0)
+    genes["phase"] = np.mod(genes["phase"], 1.0)
+    genes["strength"] = np.clip(genes["strength"], 0.1, 1.0)
+    genes["friction"] = np.clip(genes["friction"], 0.2, 1.0)
+    genes["contracted"] = np.minimum(genes["contracted"], genes["extended"])
+    # the records (records_from_genes, element for element, in float32)
+    out = Batch(batch.records.copy(), batch.muscles.copy(), batch.heads.copy(), batch.slots.copy())
+    om = out.muscles.reshape(-1, N.MUSCLE_WORDS)
+    per, dty = genes["period"].astype(f32), duty.astype(f32)
+    lng, sht = genes["extended"].astype(f32), genes["contracted"].astype(f32)
+    stg = genes["strength"].astype(f32)
+    cap = ((f32(2.0) * f32(DUTY_SPEED)) * per * np.minimum(dty, f32(1.0) - dty)).astype(f32) / f32(np.pi)
+    om[mrow, 8] = np.minimum((lng - sht).astype(f32), cap.astype(f32))
+    om[mrow, 9] = f32(1.0) / per
+    om[mrow, 10] = genes["phase"].astype(f32)
+    om[mrow, 3] = f32(1.0) / (f32(8.0) * np.maximum(lng, f32(0.05)))
+    om[mrow, 4] = f32(200.0) * stg
+    om[mrow, 5] = f32(1.0) / (f32(120.0) * stg)
+    out.records[nword] = genes["friction"].astype(f32).view(np.uint32)
+    return out
+
+
 def gene_direction(genes: dict, rng) -> dict:
     """A random direction in gene space, each gene's component scaled by the
     gene's magnitude, normalised to unit length."""

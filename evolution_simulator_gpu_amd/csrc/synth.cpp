@@ -226,6 +226,11 @@ int evo_synth_preset(int preset, evo_synth_spec* s) {
         case 3:
             s->seed = 4; s->count = 65536; s->min_nodes = 4; s->max_nodes = 10; s->min_muscles = 6; s->max_muscles = 16;
             break;
+        case 4:
+            // the bodies of preset 2 from their own seed, packed class-major
+            s->seed = 5; s->count = 3000000; s->min_nodes = 3; s->max_nodes = 32; s->node_p = 0.25f;
+            s->min_muscles = 1; s->max_muscles = 64; s->max_constraints = 64; s->sorted = 2;
+            break;
         default:
             return EVO_ERR_INVALID;
     }

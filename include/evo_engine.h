@@ -152,8 +152,8 @@ typedef struct evo_synth_spec {
 /* Fills `s` with the spec of a benchmark workload: 0 = 4,096 creatures of
  * 3-6 nodes and 2-8 muscles from seed 1; 1 = 3-12 nodes with bones + muscles
  * <= 24 from seed 2; 2 = 3-32 nodes (geometric, p = 0.25) with up to 64
- * constraints; 3 = 4-10 nodes with 6-16 muscles. `count` is then set by the
- * caller. */
+ * constraints; 3 = 4-10 nodes with 6-16 muscles; 4 = preset 2's bodies from
+ * seed 5, packed class-major. `count` is then set by the caller. */
 int evo_synth_preset(int preset, evo_synth_spec* s);
 /* Words of node and bone records and floats of muscle records the batch needs. */
 int evo_synth_sizes(const evo_synth_spec* s, uint64_t* record_words, uint64_t* muscle_floats);

@@ -108,6 +108,20 @@ def test_ascend_moves_genes_within_their_ranges():
     moved.validate()
 
 
+def test_ascend_batch_is_ascend_for_every_creature():
+    """The whole-batch mutation writes the records that mutating the
+    creatures one at a time writes, bit for bit."""
+    batch = E.synth(E.synth_spec(3, 24))
+    rng = np.random.default_rng(4)
+    grads = rng.normal(size=(24, A.STRIDE)) * np.exp(rng.normal(size=(24, 1)) * 3)
+    grads[5] = 0.0  # a creature without a gradient keeps its genes
+    one = A.ascend(batch, range(24), grads, rate=0.05)
+    every = A.ascend_batch(batch, grads, rate=0.05)
+    assert np.array_equal(one.muscles.view(np.uint32), every.muscles.view(np.uint32))
+    assert np.array_equal(one.records, every.records)
+    every.validate()
+
+
 def test_adjoint_refuses_what_it_cannot_differentiate():
     import ctypes as C
 
@@ -164,6 +178,46 @@ def test_gpu_adjoint_matches_the_dual_oracle(gpu, terrain):
         assert np.all(g[c, n:] == 0.0)
         err = np.abs(g[c, :n] - want).max() / np.abs(want).max()
         assert err < 1e-3, (c, err)
+
+
+HARD_WORLDS = {
+    "quake": dict(terrain=0.03, quake=0.05),
+    "ice": dict(patches=0.8, ground_friction=1.5),
+    "gaps": dict(gaps=0.35),
+    "hurdles": dict(hurdles=0.08),
+    "slope": dict(slope=0.1),
+    "wind_air": dict(wind=-3.0, air_retention=0.9),
+    "hard": dict(terrain=0.25, gravity=19.6, ground_friction=0.6, patches=0.8, gaps=0.35, hurdles=0.08, quake=0.05),
+}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("effect", sorted(HARD_WORLDS))
+def test_gpu_adjoint_matches_the_dual_oracle_in_every_world(gpu, effect):
+    """Every effect the adjoint differentiates, alone and all together (the
+    hard world of the full-stack workload), on 1 s trials: the gradient
+    agrees with the FP32 dual oracle's to 1e-3 of its largest component."""
+    import torch
+
+    world = E.world(duration=1.0, substeps=8, **HARD_WORLDS[effect])
+    batch = E.synth(E.synth_spec(3, 64, world=world))
+    p, run = E.run_for(batch, world)
+    pair = A.Pair(E.DeviceBatch(batch), p, run, seg=20)
+    pair.forward()
+    pair.backward()
+    torch.cuda.synchronize()
+    res = pair.db.results_numpy()
+    assert np.array_equal(pair.fitness.cpu().numpy().view(np.uint32), res["fitness"].view(np.uint32))
+    g = pair.gradient_numpy()
+    checked = 0
+    for c in range(0, 64, 4):
+        if (res["ground_hi"][c:c + 1].view(np.uint32)[0] & 32) != 0:
+            continue
+        _, want = oracle.full_gradient(batch, p, run, c, "f32")
+        err = np.abs(g[c, :want.size] - want).max() / max(np.abs(want).max(), 1e-30)
+        assert err < 1e-3, (effect, c, err)
+        checked += 1
+    assert checked >= 8
 
 
 @pytest.mark.gpu
