@@ -459,18 +459,22 @@ def from_bodies(bodies: list[dict], ids: list[int] | None = None, flags: list[in
                  np.arange(len(bodies), dtype=np.uint32))
 
 
-def class_ranges(batch: Batch, grids: tuple[int, int] = (0, 0)) -> list[tuple[int, int, int, int, int, int]]:
-    """The two body-size classes of a class-major packed batch (synthetic
-    spec sorted = 2): bodies of at most 12 nodes and 24 muscles, then the
-    rest, as (base, count, max_nodes, max_muscles, no_tendons, grid)."""
+def class_ranges(batch: Batch, grids: tuple[int, ...] = (0, 0, 0)) -> list[tuple[int, int, int, int, int, int]]:
+    """The body-size classes of a class-major packed batch (synthetic spec
+    sorted = 2): bodies of at most 12 nodes and 24 muscles, bodies of at most
+    12 nodes and more muscles, then the rest, as (base, count, max_nodes,
+    max_muscles, no_tendons, grid) for each class the batch holds; `grids[k]`
+    is class k's grid (0: the kernel's own)."""
     n = batch.nodes().astype(np.int64)
     m = batch.muscle_counts().astype(np.int64)
-    big = (n > 12) | (m > 24)
-    cut = int(np.argmax(big)) if big.any() else batch.count
-    if big[:cut].any() or not big[cut:].all():
+    cls = np.where(n > 12, 2, np.where(m > 24, 1, 0))
+    if np.any(np.diff(cls) < 0):
         raise ValueError("the batch is not packed class-major")
+    cuts = [0] + [int(np.searchsorted(cls, k)) for k in (1, 2)] + [batch.count]
+    grids = tuple(grids) + (0,) * (3 - len(grids))
     out = []
-    for k, (a, b) in enumerate(((0, cut), (cut, batch.count))):
+    for k in range(3):
+        a, b = cuts[k], cuts[k + 1]
         if b > a:
             # a range's muscle records are contiguous in packed order
             lo = int(batch.heads[a, 4])

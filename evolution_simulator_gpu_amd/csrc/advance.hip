@@ -221,7 +221,7 @@ void evo_graph_destroy(evo_graph* g) {
 
 const char* evo_build_info(void) {
     return "evo-mi355x gfx950: rates {60,120} x substeps {8,16} x worlds {flat, any, per-creature table} x sizes "
-           "{12n/8m unrolled, 12n/24m unrolled, 32n/96m looped}; recording: any world, 12n/24m unrolled or looped";
+           "{12n/8m unrolled, 12n/24m unrolled, 12n/96m muscles looped, 32n/96m looped}; recording: any world, 12n/24m unrolled or looped";
 }
 
 }  // extern "C"

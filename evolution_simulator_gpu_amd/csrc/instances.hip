@@ -80,6 +80,15 @@ Instance pick_size(const evo_run& run) {
     if (known && run.max_nodes <= static_cast<uint32_t>(kMidN) && run.max_muscles <= static_cast<uint32_t>(kMidM)) {
         return run.no_tendons ? pick_unrolled<WORLD, false>(run) : pick_unrolled<WORLD, true>(run);
     }
+    // few nodes and many muscles: nodes, bones and joints unrolled, muscles
+    // as run-time loops, which leaves the LDS of 12 nodes and so twice the
+    // waves of the looped class on a CU
+    if (known && run.max_nodes <= static_cast<uint32_t>(kMidN)) {
+        if (run.no_tendons) {
+            return instance<Cfg<R, S, WORLD, kMidN, EVO_MAX_MUSCLES, false, 64, true, false, false, true>>();
+        }
+        return instance<Cfg<R, S, WORLD, kMidN, EVO_MAX_MUSCLES, false, 64, true, true, false, true>>();
+    }
     // bodies without tendons skip the tendon words of every muscle
     if (run.no_tendons) { return instance<Cfg<R, S, WORLD, EVO_MAX_NODES, EVO_MAX_MUSCLES, false, 64, false, false>>(); }
     return instance<Cfg<R, S, WORLD, EVO_MAX_NODES, EVO_MAX_MUSCLES, false, 64, false, true>>();

@@ -49,9 +49,12 @@ constexpr uint32_t kAnyWorld = 0xffffffffu;
 constexpr uint32_t kTableWorld = 0xfffffffeu;
 
 template <uint32_t RATE, uint32_t SUB, uint32_t WORLD, int MAXN_, int MAXM_, bool RECORD, int BLOCK_, bool UNROLLED,
-          bool TENDON, bool CKPT = false>
+          bool TENDON, bool CKPT = false, bool MLOOP = false>
 struct Cfg {
     static constexpr bool unrolled = UNROLLED;
+    // muscles as run-time loops (constants in scratch) while nodes, bones
+    // and joints stay unrolled: bodies of few nodes and many muscles
+    static constexpr bool munrolled = UNROLLED && !MLOOP;
     static constexpr bool tendon = TENDON;
     // writes the adjoint's checkpoints (the forward half of a forward/backward pair)
     static constexpr bool ckpt = CKPT;
@@ -134,6 +137,25 @@ __device__ __forceinline__ void items(uint32_t bound, F&& f) {
 template <class C, int N, class F>
 __device__ __forceinline__ void all_items(F&& f) {
     if constexpr (C::unrolled) {
+        unroll<N>(f);
+    } else {
+        for (int i = 0; i < N; i++) { f(i); }
+    }
+}
+// The same over muscles, which a class may run as run-time loops alone.
+template <class C, int N, int START, int G, class F>
+__device__ __forceinline__ void mitems(uint32_t bound, F&& f) {
+    if constexpr (C::munrolled) {
+        groups_from<START, N, G>(bound, f);
+    } else {
+        const uint32_t end = bound < static_cast<uint32_t>(N) ? bound : static_cast<uint32_t>(N);
+#pragma unroll 4
+        for (uint32_t i = START; i < end; i++) { f(static_cast<int>(i)); }
+    }
+}
+template <class C, int N, class F>
+__device__ __forceinline__ void mall_items(F&& f) {
+    if constexpr (C::munrolled) {
         unroll<N>(f);
     } else {
         for (int i = 0; i < N; i++) { f(i); }
@@ -259,9 +281,9 @@ struct Cols {
     __device__ __forceinline__ float4& p(uint32_t i) const { return pv[i * C::BLOCK]; }
     __device__ __forceinline__ float2& f(uint32_t i) const { return fp[i * C::BLOCK]; }
     // muscle rhythm offsets: an LDS column in the unrolled classes, the
-    // lane's own (scratch) array in the run-time-loop class, whose LDS holds
-    // only the node state so that more of its blocks fit on a CU
-    __device__ __forceinline__ float& o(uint32_t k) const { return off[k * (C::unrolled ? C::BLOCK : 1u)]; }
+    // lane's own (scratch) array where muscles run as run-time loops, whose
+    // LDS holds only the node state so that more of its blocks fit on a CU
+    __device__ __forceinline__ float& o(uint32_t k) const { return off[k * (C::munrolled ? C::BLOCK : 1u)]; }
 };
 
 // The wave's largest node, bone and muscle counts: the scalar loop bounds.
@@ -387,7 +409,7 @@ __device__ __forceinline__ void joint_push(const Cols<C>& s, const Joint& k, flo
 template <class C>
 __device__ __forceinline__ void muscle_demands(Body<C>& b, const Cols<C>& s, const float* __restrict__ mus, float t,
                                                bool limp, const Bounds& wb) {
-    items<C, C::MAXM, 0, 1>(wb.m, [&](auto kc) {
+    mitems<C, C::MAXM, 0, 1>(wb.m, [&](auto kc) {
         const int k = kc;
         if (static_cast<uint32_t>(k) < b.muscles) {
             const float4* q = reinterpret_cast<const float4*>(mus + k * EVO_MUSCLE_WORDS);
@@ -435,7 +457,7 @@ template <class C>
 __device__ __forceinline__ void muscle_forces(Body<C>& b, const Cols<C>& s, float rec_h, float inv_me,
                                               const Bounds& wb) {
     items<C, C::MAXN, 0, 4>(wb.n, [&](auto ic) { s.f(static_cast<int>(ic)) = make_float2(0.0f, 0.0f); });
-    items<C, C::MAXM, 0, 2>(wb.m, [&](auto kc) {
+    mitems<C, C::MAXM, 0, 2>(wb.m, [&](auto kc) {
         const int k = kc;
         uint32_t nodes = b.mnodes[k];
         opaque<C>(nodes);
@@ -726,7 +748,7 @@ __device__ __forceinline__ void record_frame(const Body<C>& b, const Cols<C>& s,
                 f[EVO_MAX_NODES + b.muscles + i] = make_float2(b.normal_force[i], b.friction_force[i]);
             }
         });
-        all_items<C, C::MAXM>([&](auto kc) {
+        mall_items<C, C::MAXM>([&](auto kc) {
             const int k = kc;
             if (static_cast<uint32_t>(k) < b.muscles) { f[EVO_MAX_NODES + k] = make_float2(b.mstore[k], b.mpull[k]); }
         });
@@ -824,7 +846,7 @@ __device__ __forceinline__ void begin_creature(Body<C>& b, Trial& tr, const Cols
         b.jhi[j] = hi;
         b.jwf[j] = wf;
     });
-    all_items<C, C::MAXM>([&](auto kc) {
+    mall_items<C, C::MAXM>([&](auto kc) {
         const int k = kc;
         if (static_cast<uint32_t>(k) < b.muscles) {
             const float4* q = reinterpret_cast<const float4*>(muscles + tr.mus_off + k * EVO_MUSCLE_WORDS);
@@ -967,7 +989,7 @@ __device__ __forceinline__ void write_checkpoint(const Body<C>& b, const Trial& 
         rec[(4u * ck.an + ck.am + k) * 64u + l] = s.o(k);
         rec[(4u * ck.an + 2u * ck.am + k) * 64u + l] = ckpt_rst(ck, at, k);
     }
-    all_items<C, C::MAXM>([&](auto kc) {
+    mall_items<C, C::MAXM>([&](auto kc) {
         const int k = kc;
         if (static_cast<uint32_t>(k) < mcount) { rec[(4u * ck.an + static_cast<uint32_t>(k)) * 64u + l] = b.mstore[k]; }
     });
@@ -1100,7 +1122,7 @@ __device__ __forceinline__ void run_step(Body<C>& b, Trial& tr, const Cols<C>& s
             const float speed = (com_x - tr.rung_x) * (C::rate / static_cast<float>(C::half_s));
             const float touched = static_cast<float>(__builtin_popcount(tr.contact_bits)) / static_cast<float>(b.nodes);
             float en_mean = 0.0f;
-            items<C, C::MAXM, 0, 1>(wb.m, [&](auto kc) {
+            mitems<C, C::MAXM, 0, 1>(wb.m, [&](auto kc) {
                 if (static_cast<uint32_t>(static_cast<int>(kc)) < b.muscles) { en_mean += b.mstore[static_cast<int>(kc)]; }
             });
             en_mean /= static_cast<float>(b.muscles > 1u ? b.muscles : 1u);
@@ -1226,14 +1248,14 @@ __global__ void __launch_bounds__(C::BLOCK) advance_kernel(const uint32_t* __res
                                                           uint64_t frames_per_creature, const CkptOut ck) {
     __shared__ float4 s_pv[(C::MAXN + 1) * C::BLOCK];
     __shared__ float2 s_fp[(C::MAXN + 1) * C::BLOCK];
-    __shared__ float s_off[(C::unrolled && C::MAXM > 0 ? C::MAXM : 1) * C::BLOCK];
-    float l_off[C::unrolled ? 1 : C::MAXM];
+    __shared__ float s_off[(C::munrolled && C::MAXM > 0 ? C::MAXM : 1) * C::BLOCK];
+    float l_off[C::munrolled ? 1 : C::MAXM];
     const uint32_t lane = threadIdx.x & 63u;
     const uint64_t below = (1ull << lane) - 1ull;
     Cols<C> s;
     s.pv = s_pv + threadIdx.x;
     s.fp = s_fp + threadIdx.x;
-    if constexpr (C::unrolled) {
+    if constexpr (C::munrolled) {
         s.off = s_off + threadIdx.x;
     } else {
         s.off = l_off;

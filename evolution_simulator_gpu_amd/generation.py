@@ -73,10 +73,18 @@ class IslandGeneration:
         if candidates is None:
             if len(caps) == 1:
                 candidates = [(0,)]
-            else:
+            elif len(caps) == 2:
                 m, b = caps
                 candidates = [(0, 0), (m // 2, b // 2), (m // 4, (3 * b) // 4), ((3 * m) // 4, b // 4),
                               (m // 2, b), (m // 4, b), (m, b // 2)]
+            else:
+                # shares of the GPU, each class's grid that share of the
+                # blocks of it that fit alone; all 0: every kernel as large
+                # as fits, one after another as blocks free up
+                shares = [(1 / 3, 1 / 3, 1 / 3), (1 / 4, 1 / 2, 1 / 4), (1 / 4, 5 / 8, 1 / 8), (1 / 8, 3 / 4, 1 / 8),
+                          (3 / 8, 1 / 2, 1 / 8), (1 / 8, 5 / 8, 1 / 4)]
+                candidates = [tuple(0 for _ in caps)]
+                candidates += [tuple(max(1, int(f * c)) for f, c in zip(share, caps)) for share in shares]
         short = [N.WorldConfig.from_buffer_copy(w) for w in self.worlds]
         for w in short:
             w.duration = seconds
@@ -135,14 +143,15 @@ class PerShapeGeneration:
         world_of = np.asarray(world_of)
         n = batch.nodes().astype(np.int64)
         m = batch.muscle_counts().astype(np.int64)
-        big = ((n > 12) | (m > 24)).astype(np.int64)
+        # the body-size classes of class_ranges
+        cls = np.where(n > 12, 2, np.where(m > 24, 1, 0)).astype(np.int64)
         # packed order within each (world, class) group is kept
-        self.order = np.lexsort((np.arange(batch.count), big, world_of))
+        self.order = np.lexsort((np.arange(batch.count), cls, world_of))
         self.batch = batch.subset(self.order)
         self.db = DeviceBatch(self.batch, device)
         self.db.counter = torch.zeros(streams, dtype=torch.int32, device=device)
-        wo, cl = world_of[self.order], big[self.order]
-        key = wo * 2 + cl
+        wo, cl = world_of[self.order], cls[self.order]
+        key = wo * 3 + cl
         cuts = np.concatenate([[0], np.nonzero(np.diff(key))[0] + 1, [batch.count]])
         stride = 0
         self.launches = []

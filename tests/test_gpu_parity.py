@@ -8,6 +8,7 @@ import pytest
 
 import evolution_simulator_gpu_amd as E
 import oracle
+from evolution_simulator_gpu_amd import genome as G
 from tests.helpers import assert_bit_exact, config0_batch, flat_world
 
 pytestmark = pytest.mark.gpu
@@ -45,3 +46,27 @@ def test_large_bodies_use_looped_class(gpu):
     batch = E.synth(spec)
     got, want = _both(batch, flat_world(duration=2.0))
     assert_bit_exact(got, want, "3-32 nodes, up to 64 constraints")
+
+
+def test_few_nodes_many_muscles_class(gpu):
+    """Bodies of at most 12 nodes and more than 24 muscles run with nodes,
+    bones and joints unrolled and muscles as run-time loops: tendon-free
+    synthetic bodies in a calm and a hard world, and random genomes with
+    tendons."""
+    spec = E.synth_spec(2, 512)
+    spec.max_nodes, spec.min_muscles = 12, 25
+    batch = E.synth(spec)
+    assert batch.nodes().max() <= 12 and batch.muscle_counts().min() > 24 and batch.tendon_free()
+    got, want = _both(batch, flat_world(duration=2.0))
+    assert_bit_exact(got, want, "3-12 nodes, 25-64 muscles")
+    hard = E.world(duration=2.0, substeps=8, terrain=0.25, gravity=19.6, ground_friction=0.6, patches=0.8,
+                   gaps=0.35, hurdles=0.08, quake=0.05)
+    got, want = _both(batch, hard)
+    assert_bit_exact(got, want, "3-12 nodes, 25-64 muscles, hard world")
+    rng = np.random.default_rng(11)
+    cs = [G.random_creature(rng, int(rng.integers(3, 13)), int(rng.integers(25, 60)), id=i) for i in range(256)]
+    w = flat_world(duration=2.0)
+    tendons = G.pack(cs, w)
+    assert not tendons.tendon_free()
+    got, want = _both(tendons, w)
+    assert_bit_exact(got, want, "3-12 nodes, 25-59 muscles with tendons")
