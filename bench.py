@@ -315,8 +315,9 @@ def hard_world(duration: float):
 def full(args, torch):
     """One generation of the full stack: 3,000,000 creatures of 3-32 nodes
     in the hard world (concurrent per-class persistent kernels from a HIP
-    graph), then the forward/backward pair on the fittest `--elites` that
-    the adjoint's body classes hold (<= 12 nodes, <= 24 muscles), the
+    graph), then the forward/backward pairs on the fittest `--elites` that
+    the adjoint's body classes hold (<= 12 nodes, <= 64 muscles; one pair
+    for bodies of <= 24 muscles, one for more), the
     gradient-directed mutation of those elites, and replay capture of the
     `--replays` fittest."""
     import oracle  # the checker of the sampled gradients
@@ -358,28 +359,39 @@ def full(args, torch):
         fit = torch.nan_to_num(fit, nan=float("-inf"))
         elites = torch.topk(fit, min(args.elites, batch.count)).indices.cpu().numpy()
         fits = (nodes[elites] <= A.MAX_NODES) & (muscles[elites] <= A.MAX_MUSCLES)
-        chosen = np.sort(elites[fits])
-        eb = batch.subset(chosen)
+        many = muscles[elites] > A.MID_MUSCLES
+        # one pair per adjoint class: bodies of at most 24 muscles (unrolled
+        # classes), then of more (the many-muscle class)
+        groups = [np.sort(elites[fits & ~many]), np.sort(elites[fits & many])]
+        groups = [g for g in groups if g.size]
+        chosen = np.concatenate(groups)
+        subsets = [batch.subset(g) for g in groups]
         select_s = time.perf_counter() - t1
-        note(f"{chosen.size} of {elites.size} elites in the adjoint's classes, gathered in {select_s:.1f} s")
+        note(f"{chosen.size} of {elites.size} elites in the adjoint's classes "
+             f"({' + '.join(str(g.size) for g in groups)}), gathered in {select_s:.1f} s")
         # the forward/backward pair on them
-        pe, re = E.run_for(eb, world)
-        pair = A.Pair(E.DeviceBatch(eb), pe, re, seg=60)
+        pairs = []
+        for eb in subsets:
+            pe, re = E.run_for(eb, world)
+            pairs.append(A.Pair(E.DeviceBatch(eb), pe, re, seg=60))
         torch.cuda.synchronize()
         e = [ev() for _ in range(3)]
         e[0].record()
-        pair.forward()
+        for pair in pairs:
+            pair.forward()
         e[1].record()
-        pair.backward()
+        for pair in pairs:
+            pair.backward()
         e[2].record()
         torch.cuda.synchronize()
         pair_fwd_ms, pair_bwd_ms = e[0].elapsed_time(e[1]), e[1].elapsed_time(e[2])
-        note(f"pair on {eb.count} elites: forward {pair_fwd_ms:.0f} ms, backward {pair_bwd_ms:.0f} ms")
-        grads = pair.gradient_numpy()
+        note(f"pairs on {chosen.size} elites: forward {pair_fwd_ms:.0f} ms, backward {pair_bwd_ms:.0f} ms")
+        grads = [pair.gradient_numpy() for pair in pairs]
         # gradient-directed mutation of every elite with a finite gradient
         t2 = time.perf_counter()
-        finite = np.isfinite(grads).all(axis=1)
-        mutated = A.ascend_batch(eb, np.where(finite[:, None], grads, 0.0), rate=0.02)
+        finite = [np.isfinite(g).all(axis=1) for g in grads]
+        mutated = [A.ascend_batch(eb, np.where(f[:, None], g, 0.0), rate=0.02)
+                   for eb, g, f in zip(subsets, grads, finite)]
         mutate_s = time.perf_counter() - t2
         # replay capture of the fittest
         e = [ev() for _ in range(2)]
@@ -393,17 +405,22 @@ def full(args, torch):
         rows.append((wall, forward_ms, select_s, pair_fwd_ms, pair_bwd_ms, mutate_s, replay_ms))
     res = gen.results()
     substeps = int((res["ground_hi"].view(np.uint32) >> 16).astype(np.int64).sum()) * world.substeps
-    pres = pair.db.results_numpy()
-    elite_substeps = int((pres["ground_hi"].view(np.uint32) >> 16).astype(np.int64).sum()) * world.substeps
-    same = bool(np.array_equal(pres["fitness"].view(np.uint32), res["fitness"][chosen].view(np.uint32)))
-    # sampled check of the elites' gradients against the FP32 dual oracle
+    elite_substeps = 0
+    same = True
+    rel32 = []
     rng = np.random.default_rng(12)
-    ok = np.nonzero(finite & ((pres["ground_hi"].view(np.uint32) & 32) == 0))[0]
-    sample = rng.choice(ok, min(64, ok.size), replace=False)
-    dirs = [rng.normal(size=oracle.param_count(eb, int(c))) for c in sample]
-    _, d32 = oracle.directional(eb, pe, re, sample, dirs, "f32", threads=16)
-    mine = np.array([grads[c, :len(d)] @ d for c, d in zip(sample, dirs)])
-    rel32 = np.abs(mine - d32) / np.maximum(np.abs(d32), 1e-12)
+    for g, pair, grad, fin in zip(groups, pairs, grads, finite):
+        pres = pair.db.results_numpy()
+        elite_substeps += int((pres["ground_hi"].view(np.uint32) >> 16).astype(np.int64).sum()) * world.substeps
+        same = same and bool(np.array_equal(pres["fitness"].view(np.uint32), res["fitness"][g].view(np.uint32)))
+        # sampled check of the elites' gradients against the FP32 dual oracle
+        ok = np.nonzero(fin & ((pres["ground_hi"].view(np.uint32) & 32) == 0))[0]
+        sample = rng.choice(ok, min(32, ok.size), replace=False)
+        dirs = [rng.normal(size=oracle.param_count(pair.db.batch, int(c))) for c in sample]
+        _, d32 = oracle.directional(pair.db.batch, pair.p, pair.run, sample, dirs, "f32", threads=16)
+        mine = np.array([grad[c, :len(d)] @ d for c, d in zip(sample, dirs)])
+        rel32.extend(np.abs(mine - d32) / np.maximum(np.abs(d32), 1e-12))
+    rel32 = np.array(rel32)
     wall, forward_ms, select_s, pf, pb, mutate_s, replay_ms = (float(np.mean(x)) for x in zip(*rows))
     line = {
         "metric": "creature-substeps/s at 3M creatures, 20 s trials, full stack (hard world, adjoint on elites, "
@@ -431,18 +448,19 @@ def full(args, torch):
             "forward_creature_substeps_per_s": substeps / (forward_ms / 1e3),
             "elites": int(elites.size),
             "elites_with_gradients": int(chosen.size),
+            "elites_per_adjoint_class": [int(g.size) for g in groups],
             "elite_coverage": float(chosen.size / max(elites.size, 1)),
             "elite_select_gather_s": select_s,
             "pair_forward_ms": pf,
             "pair_backward_ms": pb,
             "pair_over_plain_forward_of_elites": (pf + pb) / pf,
             "elite_fitness_bit_exact": same,
-            "elite_gradients_beyond_float_range": int((~finite).sum()),
+            "elite_gradients_beyond_float_range": int(sum((~f).sum() for f in finite)),
             "mutation_s": mutate_s,
-            "mutated_records_valid": bool(mutated.validate() is None),
+            "mutated_records_valid": all(m.validate() is None for m in mutated),
             "replays": replays,
             "replay_ms": replay_ms,
-            "elite_gradients_vs_fp32_dual_oracle": {"sample": int(sample.size),
+            "elite_gradients_vs_fp32_dual_oracle": {"sample": int(rel32.size),
                                                     "median_rel": finite_or_none(np.median(rel32)),
                                                     "within_1e-3": float((rel32 < 1e-3).mean())},
             "tuning": tuning,

@@ -186,7 +186,8 @@ SIGNATURES = {
     "evo_wild_world": ([C.POINTER(WorldConfig), C.POINTER(WildMix), C.POINTER(WorldConfig)], C.c_int),
     "evo_island_worlds": ([_u64, _u32, C.POINTER(WorldConfig), _P], C.c_int),
     "evo_splitmix_finalize": ([_u64], _u64),
-    "evo_adjoint_workspace_bytes": ([_u32, _u32, _u32, _u32], _u64),
+    "evo_device_error": ([], C.c_char_p),
+    "evo_adjoint_workspace_bytes": ([C.POINTER(Run), _u32, _u32, _u32], _u64),
     "evo_adjoint_forward_device": ([_P, _P, _P, _P, _P, C.POINTER(Params), C.POINTER(Run), _u32, _P, _u64, _P],
                                    C.c_int),
     "evo_adjoint_device": ([_P, _P, _P, _P, C.POINTER(Params), C.POINTER(Run), _u32, _P, _P, _P, _u64, _u32, _P],
@@ -230,6 +231,13 @@ def lib() -> C.CDLL:
         path = Path(os.environ.get("EVO_LIB", LIB_PATH))
         if not path.exists():
             raise RuntimeError(f"{path} is missing: build it with `python -m evolution_simulator_gpu_amd.build`")
+        # torch carries its own HIP runtime (torch/lib/libamdhip64.so, soname
+        # libamdhip64.so.7). Loaded first, it is the one this library binds
+        # to; loaded after this library, it would be a second runtime in the
+        # process, and launches here would not see the device torch set up
+        # (hipErrorNoDevice) nor share its memory safely.
+        import torch  # noqa: F401
+
         handle = C.CDLL(str(path))
         for name, (argtypes, restype) in SIGNATURES.items():
             fn = getattr(handle, name)

@@ -21,8 +21,12 @@ import numpy as np
 from . import _native as N
 from .batch import Batch, DeviceBatch, concat
 
-STRIDE = 6 * 24 + 12
-MAX_NODES, MAX_MUSCLES = 12, 24
+# EVO_ADJ_MAX_NODES, EVO_ADJ_MAX_MUSCLES, EVO_ADJ_STRIDE
+MAX_NODES, MAX_MUSCLES = 12, 64
+STRIDE = 6 * MAX_MUSCLES + MAX_NODES
+# bodies of at most 12 nodes and 24 muscles run in the adjoint's unrolled
+# classes, larger ones (up to MAX_MUSCLES) in its many-muscle class
+MID_MUSCLES = 24
 PER_MUSCLE = ("amplitude", "inv_period", "phase", "hill", "cap", "inv_capacity")
 DUTY_SPEED = 24.0  # the rhythm's speed limit behind the amplitude cap (m/s)
 
@@ -38,10 +42,12 @@ class Pair:
         import torch
 
         if run.max_nodes > MAX_NODES or run.max_muscles > MAX_MUSCLES:
-            raise ValueError("the adjoint handles bodies of at most 12 nodes and 24 muscles")
+            raise ValueError(f"the adjoint handles bodies of at most {MAX_NODES} nodes and {MAX_MUSCLES} muscles")
         self.db, self.p, self.run, self.seg = db, p, run, seg
         n = db.batch.count
-        size = N.lib().evo_adjoint_workspace_bytes(n, p.steps, run.substeps, seg)
+        size = N.lib().evo_adjoint_workspace_bytes(C.byref(run), n, p.steps, seg)
+        if size == 0:
+            raise ValueError("the adjoint does not handle this run")
         self.workspace = torch.empty(size, dtype=torch.uint8, device=db.device)
         self.grads = torch.zeros(n * STRIDE, dtype=torch.float32, device=db.device)
         self.fitness = torch.zeros(n, dtype=torch.float32, device=db.device)
@@ -58,7 +64,7 @@ class Pair:
             self.db.results.data_ptr(), self.db.counter.data_ptr(), C.byref(self.p), C.byref(self.run), self.seg,
             self.workspace.data_ptr(), self.workspace.numel(), C.c_void_p(s.cuda_stream))
         if rc != N.EVO_OK:
-            raise RuntimeError(f"evo_adjoint_forward_device failed: {rc}")
+            raise RuntimeError(f"evo_adjoint_forward_device failed: {rc} ({N.lib().evo_device_error().decode()})")
 
     def backward(self, stream=None, checkpointed: bool = True) -> None:
         s = self._stream(stream)
@@ -68,7 +74,7 @@ class Pair:
             self.fitness.data_ptr(), self.workspace.data_ptr(), self.workspace.numel(), int(checkpointed),
             C.c_void_p(s.cuda_stream))
         if rc != N.EVO_OK:
-            raise RuntimeError(f"evo_adjoint_device failed: {rc}")
+            raise RuntimeError(f"evo_adjoint_device failed: {rc} ({N.lib().evo_device_error().decode()})")
 
     def gradient_numpy(self) -> np.ndarray:
         return self.grads.cpu().numpy().reshape(-1, STRIDE)

@@ -13,7 +13,7 @@ namespace {
 
 using evo::dev::adj::AdjointArgs;
 using evo::dev::adj::AdjointLauncher;
-using Large = evo::dev::adj::Words<evo::dev::adj::kLargeNodes, evo::dev::adj::kLargeMuscles>;
+using evo::dev::adj::Class;
 
 uint32_t segments(uint32_t steps, uint32_t seg) { return (steps + seg - 1u) / seg; }
 uint64_t lanes(uint32_t count) { return (static_cast<uint64_t>(count) + 63u) & ~static_cast<uint64_t>(63u); }
@@ -26,6 +26,40 @@ evo::dev::Instance pick_forward(const evo_run& run) {
     return evo::dev::Instance{nullptr, nullptr};
 }
 
+// Workspace words per lane of a class.
+uint64_t lane_words(Class k, uint32_t steps, uint32_t substeps, uint32_t seg) {
+    using namespace evo::dev::adj;
+    const auto words = [&](auto w) {
+        using L = decltype(w);
+        return static_cast<uint64_t>(segments(steps, seg)) * L::ckpt +
+               static_cast<uint64_t>(seg) * substeps * (L::slot + L::tape) + static_cast<uint64_t>(seg) * L::step +
+               L::columns;
+    };
+    switch (k) {
+        case Class::small: return words(Words<kSmallNodes, kSmallMuscles>{});
+        case Class::mid: return words(Words<kMidNodes, kMidMuscles>{});
+        case Class::many: return words(Words<kManyNodes, kManyMuscles>{});
+        default: return 0u;
+    }
+}
+
+// The workspace's regions for a run's class.
+evo::dev::adj::Regions class_regions(Class k, float* ws, uint64_t stride, uint32_t segs, uint32_t seg,
+                                     uint32_t substeps, uint32_t& an, uint32_t& am) {
+    using namespace evo::dev::adj;
+    switch (k) {
+        case Class::small:
+            an = kSmallNodes, am = kSmallMuscles;
+            return regions<kSmallNodes, kSmallMuscles>(ws, stride, segs, seg, substeps);
+        case Class::mid:
+            an = kMidNodes, am = kMidMuscles;
+            return regions<kMidNodes, kMidMuscles>(ws, stride, segs, seg, substeps);
+        default:
+            an = kManyNodes, am = kManyMuscles;
+            return regions<kManyNodes, kManyMuscles>(ws, stride, segs, seg, substeps);
+    }
+}
+
 // The checks both halves of a pair make.
 int check_pair(const uint32_t* d_records, const float* d_muscles, const uint32_t* d_heads,
                const evo_result* d_results, const evo_params* p, const evo_run* run, uint32_t seg,
@@ -34,13 +68,8 @@ int check_pair(const uint32_t* d_records, const float* d_muscles, const uint32_t
     if (seg == 0u || p->steps == 0u || run->record || (run->flags & ~evo::dev::adj::kFlags) != 0u) {
         return EVO_ERR_INVALID;
     }
-    if (run->max_nodes == 0u || run->max_nodes > static_cast<uint32_t>(EVO_ADJ_MAX_NODES) ||
-        run->max_muscles > static_cast<uint32_t>(EVO_ADJ_MAX_MUSCLES)) {
-        return EVO_ERR_INVALID;
-    }
-    if (workspace_bytes < evo_adjoint_workspace_bytes(p->count, p->steps, run->substeps, seg)) {
-        return EVO_ERR_INVALID;
-    }
+    if (evo::dev::adj::body_class(*run) == Class::none) { return EVO_ERR_INVALID; }
+    if (workspace_bytes < evo_adjoint_workspace_bytes(run, p->count, p->steps, seg)) { return EVO_ERR_INVALID; }
     if ((reinterpret_cast<uintptr_t>(d_workspace) & 15u) != 0u || (reinterpret_cast<uintptr_t>(d_heads) & 15u) != 0u) {
         return EVO_ERR_INVALID;
     }
@@ -59,13 +88,9 @@ AdjointLauncher pick(const evo_run& run) {
 
 extern "C" {
 
-uint64_t evo_adjoint_workspace_bytes(uint32_t count, uint32_t steps, uint32_t substeps, uint32_t seg) {
-    if (seg == 0u) { return 0u; }
-    // the largest class's records bound every class's
-    const uint64_t words = static_cast<uint64_t>(segments(steps, seg)) * Large::ckpt +
-                           static_cast<uint64_t>(seg) * substeps * (Large::slot + Large::tape) +
-                           static_cast<uint64_t>(seg) * Large::step + Large::columns;
-    return words * lanes(count) * sizeof(float);
+uint64_t evo_adjoint_workspace_bytes(const evo_run* run, uint32_t count, uint32_t steps, uint32_t seg) {
+    if (!run || seg == 0u) { return 0u; }
+    return lane_words(evo::dev::adj::body_class(*run), steps, run->substeps, seg) * lanes(count) * sizeof(float);
 }
 
 int evo_adjoint_forward_device(const uint32_t* d_records, const float* d_muscles, const uint32_t* d_heads,
@@ -79,22 +104,14 @@ int evo_adjoint_forward_device(const uint32_t* d_records, const float* d_muscles
     if (p->count == 0u) { return EVO_OK; }
     const uint64_t stride = lanes(p->count);
     const uint32_t segs = segments(p->steps, seg);
-    float* ws = static_cast<float*>(d_workspace);
-    evo::dev::CkptOut ck;
-    if (evo::dev::adj::small_class(*run)) {
-        using namespace evo::dev::adj;
-        const Regions r = regions<kSmallNodes, kSmallMuscles>(ws, stride, segs, seg, run->substeps);
-        ck = evo::dev::CkptOut{r.ckpt, r.cols, stride, seg, kSmallNodes, kSmallMuscles};
-    } else {
-        using namespace evo::dev::adj;
-        const Regions r = regions<kLargeNodes, kLargeMuscles>(ws, stride, segs, seg, run->substeps);
-        ck = evo::dev::CkptOut{r.ckpt, r.cols, stride, seg, kLargeNodes, kLargeMuscles};
-    }
+    uint32_t an = 0u, am = 0u;
+    const evo::dev::adj::Regions r = class_regions(evo::dev::adj::body_class(*run), static_cast<float*>(d_workspace),
+                                                   stride, segs, seg, run->substeps, an, am);
+    const evo::dev::CkptOut ck{r.ckpt, r.cols, stride, seg, an, am};
     const evo::dev::WorldTable none{nullptr, nullptr, nullptr};
     const hipError_t e = in.launch(d_records, d_muscles, reinterpret_cast<const uint4*>(d_heads), d_results, d_counter,
                                    *p, run->flags, none, nullptr, 0u, ck, reinterpret_cast<hipStream_t>(stream), 0);
-    if (e == hipErrorOutOfMemory) { return EVO_ERR_OOM; }
-    return e == hipSuccess ? EVO_OK : EVO_ERR_DEVICE;
+    return evo::dev::device_code(e);
 }
 
 int evo_adjoint_device(const uint32_t* d_records, const float* d_muscles, const uint32_t* d_heads,
@@ -122,8 +139,7 @@ int evo_adjoint_device(const uint32_t* d_records, const float* d_muscles, const 
     x.stride = lanes(p->count);
     x.checkpointed = checkpointed != 0u ? 1u : 0u;
     const hipError_t e = launch(x, reinterpret_cast<hipStream_t>(stream));
-    if (e == hipErrorOutOfMemory) { return EVO_ERR_OOM; }
-    return e == hipSuccess ? EVO_OK : EVO_ERR_DEVICE;
+    return evo::dev::device_code(e);
 }
 
 }  // extern "C"

@@ -2,7 +2,8 @@
 // chosen by EVO_RATE and EVO_SUB at compile time: per world (the calm flat
 // world, or any world of the differentiated effects read from a scalar
 // register) a class for bodies of up to 10 nodes and 16 muscles without
-// tendons, and one for up to 12 nodes and 24 muscles.
+// tendons, one for up to 12 nodes and 24 muscles and one for up to 12 nodes
+// and 64 muscles, with and without tendons.
 #include <hip/hip_runtime.h>
 
 #include "adjoint_kernel.hpp"
@@ -37,10 +38,19 @@ hipError_t launch(const AdjointArgs& x, hipStream_t stream) {
 template <uint32_t WORLD>
 AdjointLauncher pick_world(const evo_run& run) {
     constexpr uint32_t R = EVO_RATE, S = EVO_SUB;
-    if (small_class(run)) {
-        return &launch<Cfg<R, S, WORLD, kSmallNodes, kSmallMuscles, false, 64, true, false>>;
+    switch (body_class(run)) {
+        case Class::small:
+            return &launch<Cfg<R, S, WORLD, kSmallNodes, kSmallMuscles, false, 64, true, false>>;
+        case Class::mid:
+            return &launch<Cfg<R, S, WORLD, kMidNodes, kMidMuscles, false, 64, true, true>>;
+        case Class::many:
+            if (run.no_tendons) {
+                return &launch<Cfg<R, S, WORLD, kManyNodes, kManyMuscles, false, 64, true, false, false, true>>;
+            }
+            return &launch<Cfg<R, S, WORLD, kManyNodes, kManyMuscles, false, 64, true, true, false, true>>;
+        default:
+            return nullptr;
     }
-    return &launch<Cfg<R, S, WORLD, kLargeNodes, kLargeMuscles, false, 64, true, true>>;
 }
 
 }  // namespace

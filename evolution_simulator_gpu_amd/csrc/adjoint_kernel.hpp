@@ -142,8 +142,8 @@ struct Tape {
 
 };
 
-// The adjoints carried in registers: of the energy stores and of the
-// step's demands.
+// The adjoints of the energy stores and of the step's demands (in registers;
+// in scratch where muscles run as run-time loops).
 template <class C>
 struct Grad {
     float as[C::MAXM], ad[C::MAXM];
@@ -227,8 +227,9 @@ __device__ __forceinline__ void refresh(Body<C>& b) {
         opq(b.jhi[j]);
         opq(b.jwf[j]);
     });
-    if constexpr (MUSCLES) {
-    all_items<C, C::MAXM>([&](auto kc) {
+    // (muscles run as run-time loops keep their constants in scratch)
+    if constexpr (MUSCLES && C::munrolled) {
+    mall_items<C, C::MAXM>([&](auto kc) {
         const int k = kc;
         opq(b.mnodes[k]);
         opq(b.maa[k]);
@@ -675,7 +676,7 @@ __device__ __forceinline__ void add4(const ACols<C>& a, uint32_t a0, uint32_t a1
 template <class C>
 __device__ __forceinline__ void muscles_back(const Body<C>& b, const Cols<C>& s, const ACols<C>& a, Grad<C>& g,
                                              float rec_h, float inv_me, const Bounds& wb) {
-    items<C, C::MAXM, 0, 1>(wb.m, [&](auto kc) {
+    mitems<C, C::MAXM, 0, 1>(wb.m, [&](auto kc) {
         const int k = kc;
         uint32_t nodes = b.mnodes[k];
         opaque<C>(nodes);
@@ -801,7 +802,7 @@ template <class C>
 __device__ __forceinline__ void demands_back(const Body<C>& b, const Cols<C>& s, const ACols<C>& a, Grad<C>& g,
                                              const float* __restrict__ mus, float t, bool sensing,
                                              const Bounds& wb) {
-    items<C, C::MAXM, 0, 1>(wb.m, [&](auto kc) {
+    mitems<C, C::MAXM, 0, 1>(wb.m, [&](auto kc) {
         const int k = kc;
         const float bd = g.ad[k];
         g.ad[k] = 0.0f;
@@ -833,7 +834,7 @@ __device__ __forceinline__ void demands_back(const Body<C>& b, const Cols<C>& s,
 template <class C>
 __device__ __forceinline__ void demands(Body<C>& b, const ACols<C>& a, const float* __restrict__ mus, float t,
                                         bool sensing, const Bounds& wb) {
-    items<C, C::MAXM, 0, 1>(wb.m, [&](auto kc) {
+    mitems<C, C::MAXM, 0, 1>(wb.m, [&](auto kc) {
         const int k = kc;
         if (static_cast<uint32_t>(k) < b.muscles) {
             const float4* q = reinterpret_cast<const float4*>(mus + k * EVO_MUSCLE_WORDS);
@@ -866,7 +867,7 @@ __device__ __forceinline__ void rescale(const Body<C>& b, const ACols<C>& a, Gra
         const float v = fmaxf(fmaxf(fabsf(A.x), fabsf(A.y)), fmaxf(fabsf(A.z), fabsf(A.w)));
         m = static_cast<uint32_t>(i) < b.nodes ? fmaxf(m, v) : m;
     });
-    items<C, C::MAXM, 0, 4>(wb.m, [&](auto kc) {
+    mitems<C, C::MAXM, 0, 4>(wb.m, [&](auto kc) {
         const int k = kc;
         m = static_cast<uint32_t>(k) < b.muscles ? fmaxf(m, fabsf(g.as[k])) : m;
     });
@@ -882,7 +883,7 @@ __device__ __forceinline__ void rescale(const Body<C>& b, const ACols<C>& a, Gra
             A.w *= sc;
             r = A;
         });
-        all_items<C, C::MAXM>([&](auto kc) {
+        mall_items<C, C::MAXM>([&](auto kc) {
             const int k = kc;
             g.as[k] *= sc;
             g.ad[k] *= sc;
@@ -980,7 +981,7 @@ struct Work {
     // f(k, word) for the wave's muscles, over words `first` + k
     template <class F>
     __device__ __forceinline__ void muscle_words(float* base, int first, const Bounds& wb, F&& f) const {
-        items<C, C::MAXM, 0, 4>(wb.m, [&](auto kc) { f(static_cast<int>(kc), word(base, first + static_cast<int>(kc))); });
+        mitems<C, C::MAXM, 0, 4>(wb.m, [&](auto kc) { f(static_cast<int>(kc), word(base, first + static_cast<int>(kc))); });
     }
 
     __device__ __forceinline__ void put_slot(uint32_t i, const Body<C>& b, const Cols<C>& s, const Bounds& wb) const {
@@ -1025,9 +1026,11 @@ struct Work {
 // The lane's muscle constants in device memory (node word, anchors, Hill
 // factor, cap, inverse capacity, tendon stiffness and slack; eight rows a
 // muscle): the reverse holds them in registers only while the muscles run.
+// A class whose muscles run as run-time loops keeps them in scratch instead.
 template <class C>
 __device__ __forceinline__ void put_muscles(const ACols<C>& a, const Body<C>& b) {
-    all_items<C, C::MAXM>([&](auto kc) {
+    if constexpr (!C::munrolled) { return; }
+    mall_items<C, C::MAXM>([&](auto kc) {
         const int k = kc;
         const int w = ACols<C>::kMus + 8 * k;
         a.row(w) = bits_f(b.mnodes[k]);
@@ -1046,11 +1049,12 @@ __device__ __forceinline__ void put_muscles(const ACols<C>& a, const Body<C>& b)
 // where they are written.
 template <class C>
 __device__ __forceinline__ void get_muscles(const ACols<C>& a, Body<C>& b, const Bounds& wb) {
+    if constexpr (!C::munrolled) { return; }
     using gfloat = __attribute__((address_space(1))) float;
     const gfloat* base = (const gfloat*)a.cb;
     asm volatile("" : "+s"(base));
     const auto at = [&](int w) { return base[w * 64 + static_cast<int>(a.lane)]; };
-    items<C, C::MAXM, 0, 2>(wb.m, [&](auto kc) {
+    mitems<C, C::MAXM, 0, 2>(wb.m, [&](auto kc) {
         const int k = kc;
         const int w = ACols<C>::kMus + 8 * k;
         b.mnodes[k] = f_bits(at(w));
@@ -1098,8 +1102,15 @@ __global__ void __launch_bounds__(64) adjoint_kernel(const uint32_t* __restrict_
     s.fp = s_fp + lane;
     // the block's device-memory columns: offsets, reset times, gradient sums
     float* const cb = rhythm + static_cast<uint64_t>(blockIdx.x) * (Words<C::MAXN, C::MAXM>::columns * 64u);
-    // (the simulator's set-up clears the offsets through this pointer)
-    s.off = cb + lane;
+    // (the simulator's set-up clears the offsets through this pointer; with
+    // muscles as run-time loops its offsets have a stride of one, so it
+    // clears a private array the adjoint does not read)
+    float l_off[C::munrolled ? 1 : C::MAXM];
+    if constexpr (C::munrolled) {
+        s.off = cb + lane;
+    } else {
+        s.off = l_off;
+    }
     ACols<C> a;
     a.a = s_adj + lane;
     a.cb = cb;
@@ -1117,13 +1128,13 @@ __global__ void __launch_bounds__(64) adjoint_kernel(const uint32_t* __restrict_
         b.nodes = b.nodes < static_cast<uint32_t>(C::MAXN) ? b.nodes : static_cast<uint32_t>(C::MAXN);
         b.bones = b.nodes - 1u;
         b.muscles = 0u;
-        all_items<C, C::MAXM>([&](auto kc) {
+        mall_items<C, C::MAXM>([&](auto kc) {
             b.mnodes[static_cast<int>(kc)] = C::trash | (C::trash << 8u) | (C::trash << 16u) | (C::trash << 24u);
         });
     }
     bool sensor = false;
     for (uint32_t k = 0; k < b.muscles; k++) { sensor = sensor || f_bits(mus[k * EVO_MUSCLE_WORDS + 16]) != EVO_NONE; }
-    all_items<C, C::MAXM>([&](auto kc) {
+    mall_items<C, C::MAXM>([&](auto kc) {
         a.off(static_cast<int>(kc)) = 0.0f;
         a.r(static_cast<int>(kc)) = -1.0f;
     });
@@ -1163,7 +1174,7 @@ __global__ void __launch_bounds__(64) adjoint_kernel(const uint32_t* __restrict_
     for (int q = 0; q < 6 * C::MAXM + C::MAXN; q++) { *a.sum(q) = 0.0f; }
     drain();
     a.at(C::trash) = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    all_items<C, C::MAXM>([&](auto kc) {
+    mall_items<C, C::MAXM>([&](auto kc) {
         const int k = kc;
         g.as[k] = g.ad[k] = 0.0f;
     });

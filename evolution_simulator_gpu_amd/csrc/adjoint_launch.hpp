@@ -12,16 +12,30 @@ namespace evo {
 namespace dev {
 namespace adj {
 
-// Body classes: the largest node and muscle counts each unrolls to. The
-// large class is the adjoint's limit (EVO_ADJ_MAX_NODES, EVO_ADJ_MAX_MUSCLES).
+// Body classes: the largest node and muscle counts of each. The small and
+// mid classes unroll every loop (constants in registers); the many-muscle
+// class unrolls nodes, bones and joints and runs the muscles as run-time
+// loops. The many-muscle class is the adjoint's limit (EVO_ADJ_MAX_NODES,
+// EVO_ADJ_MAX_MUSCLES).
 constexpr int kSmallNodes = 10, kSmallMuscles = 16;
-constexpr int kLargeNodes = EVO_ADJ_MAX_NODES, kLargeMuscles = EVO_ADJ_MAX_MUSCLES;
+constexpr int kMidNodes = 12, kMidMuscles = 24;
+constexpr int kManyNodes = EVO_ADJ_MAX_NODES, kManyMuscles = EVO_ADJ_MAX_MUSCLES;
+
+enum class Class { none, small, mid, many };
 
 // The class of a run's bodies: bodies of at most 10 nodes and 16 muscles
-// without tendons run in the small class.
-inline bool small_class(const evo_run& run) {
-    return run.no_tendons && run.max_nodes <= static_cast<uint32_t>(kSmallNodes) &&
-           run.max_muscles <= static_cast<uint32_t>(kSmallMuscles);
+// without tendons run in the small class, bodies of at most 12 nodes in the
+// mid class up to 24 muscles and in the many-muscle class beyond.
+inline Class body_class(const evo_run& run) {
+    if (run.max_nodes == 0u || run.max_nodes > static_cast<uint32_t>(kManyNodes) ||
+        run.max_muscles > static_cast<uint32_t>(kManyMuscles)) {
+        return Class::none;
+    }
+    if (run.no_tendons && run.max_nodes <= static_cast<uint32_t>(kSmallNodes) &&
+        run.max_muscles <= static_cast<uint32_t>(kSmallMuscles)) {
+        return Class::small;
+    }
+    return run.max_muscles <= static_cast<uint32_t>(kMidMuscles) ? Class::mid : Class::many;
 }
 
 // Workspace words per lane: a checkpoint (node state, stores, rhythm

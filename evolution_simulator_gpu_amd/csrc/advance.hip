@@ -35,10 +35,7 @@ int grid_cap_from_env() {
     return cap;
 }
 
-int to_code(hipError_t e) {
-    if (e == hipErrorOutOfMemory) { return EVO_ERR_OOM; }
-    return e == hipSuccess ? EVO_OK : EVO_ERR_DEVICE;
-}
+int to_code(hipError_t e) { return evo::dev::device_code(e); }
 
 int check_args(const uint32_t* d_records, const float* d_muscles, const uint32_t* d_heads, const evo_result* d_results,
                const uint32_t* d_counter, const evo_params* p, const evo_run* run) {
@@ -89,6 +86,19 @@ Side* side_streams() {
 }
 
 }  // namespace
+
+namespace evo {
+namespace dev {
+namespace {
+thread_local hipError_t last_failure = hipSuccess;
+}
+int device_code(hipError_t e) {
+    if (e == hipSuccess) { return EVO_OK; }
+    last_failure = e;
+    return e == hipErrorOutOfMemory ? EVO_ERR_OOM : EVO_ERR_DEVICE;
+}
+}  // namespace dev
+}  // namespace evo
 
 extern "C" {
 
@@ -218,6 +228,8 @@ void evo_graph_destroy(evo_graph* g) {
     hipGraphDestroy(g->graph);
     delete g;
 }
+
+const char* evo_device_error(void) { return hipGetErrorName(evo::dev::last_failure); }
 
 const char* evo_build_info(void) {
     return "evo-mi355x gfx950: rates {60,120} x substeps {8,16} x worlds {flat, any, per-creature table} x sizes "

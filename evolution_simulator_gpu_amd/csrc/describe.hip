@@ -23,6 +23,7 @@
 #include <cstdint>
 
 #include "../../include/evo_engine.h"
+#include "launch.hpp"
 
 namespace {
 
@@ -158,6 +159,5 @@ extern "C" int evo_describe_device(const uint32_t* d_records, const uint32_t* d_
                        reinterpret_cast<const uint4*>(d_heads), d_results, base, count, steps,
                        static_cast<float>(rate), layout, d_out);
     const hipError_t e = hipGetLastError();
-    if (e == hipErrorOutOfMemory) { return EVO_ERR_OOM; }
-    return e == hipSuccess ? EVO_OK : EVO_ERR_DEVICE;
+    return evo::dev::device_code(e);
 }

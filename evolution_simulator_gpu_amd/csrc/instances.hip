@@ -9,6 +9,7 @@
 
 #include <algorithm>
 
+#include "../../include/evo_engine.h"
 #include "launch.hpp"
 #include "sim_kernel.hpp"
 
@@ -101,8 +102,18 @@ Instance pick_size(const evo_run& run) {
 Instance EVO_PICK_CKPT(const evo_run& run) {
     constexpr uint32_t R = EVO_RATE, S = EVO_SUB;
     if (run.record || run.max_nodes == 0u || run.max_nodes > static_cast<uint32_t>(kMidN) ||
-        run.max_muscles > static_cast<uint32_t>(kMidM)) {
+        run.max_muscles > static_cast<uint32_t>(EVO_ADJ_MAX_MUSCLES)) {
         return Instance{nullptr, nullptr};
+    }
+    if (run.max_muscles > static_cast<uint32_t>(kMidM)) {
+        // the adjoint's many-muscle class: muscles as run-time loops
+        constexpr int M = EVO_ADJ_MAX_MUSCLES;
+        if (run.flags == kFlat) {
+            return run.no_tendons ? instance<Cfg<R, S, kFlat, kMidN, M, false, 64, true, false, true, true>>()
+                                  : instance<Cfg<R, S, kFlat, kMidN, M, false, 64, true, true, true, true>>();
+        }
+        return run.no_tendons ? instance<Cfg<R, S, kAnyWorld, kMidN, M, false, 64, true, false, true, true>>()
+                              : instance<Cfg<R, S, kAnyWorld, kMidN, M, false, 64, true, true, true, true>>();
     }
     if (run.flags == kFlat) {
         return run.no_tendons ? instance<Cfg<R, S, kFlat, kMidN, kMidM, false, 128, true, false, true>>()

@@ -11,6 +11,11 @@
 namespace evo {
 namespace dev {
 
+// The entry points' return code for a HIP status (EVO_OK, EVO_ERR_OOM or
+// EVO_ERR_DEVICE); a failure is kept for evo_device_error.
+int device_code(hipError_t e);
+
+
 // The worlds of a batch whose creatures run in different worlds: world k's
 // launch parameters and effect bits, and the world of each packed creature.
 // All three live in device memory; `params` is null for a one-world batch.

@@ -243,13 +243,16 @@ void evo_graph_destroy(evo_graph* graph);
  * 10, 3, 4, 5), then per node i its friction (node record word 2) at
  * 6M + i; zeros past a creature's own parameters. */
 #define EVO_ADJ_MAX_NODES 12
-#define EVO_ADJ_MAX_MUSCLES 24
+#define EVO_ADJ_MAX_MUSCLES 64
 #define EVO_ADJ_STRIDE (6u * EVO_ADJ_MAX_MUSCLES + EVO_ADJ_MAX_NODES)
 
-/* Device workspace of an adjoint launch: per creature a checkpoint every
- * `seg` steps, the start state of every substep of one segment and the
- * rhythm state of every step of one segment. */
-uint64_t evo_adjoint_workspace_bytes(uint32_t count, uint32_t steps, uint32_t substeps, uint32_t seg);
+/* Device workspace of a forward/backward pair over `count` creatures of
+ * `steps` steps with `run` (its substeps and the body-size class its
+ * max_nodes, max_muscles and no_tendons select): per creature a checkpoint
+ * every `seg` steps, the start state and tape of every substep of one
+ * segment, the rhythm state of every step of one segment and the creature's
+ * columns. 0 for a run the adjoint does not handle or seg = 0. */
+uint64_t evo_adjoint_workspace_bytes(const evo_run* run, uint32_t count, uint32_t steps, uint32_t seg);
 
 /* The forward half of a forward/backward pair (no counterpart in the
  * reference, which has no gradients): evo_advance_device on the batch (the
@@ -452,6 +455,12 @@ uint32_t evo_max_muscles(const uint32_t* heads, uint32_t base, uint32_t count);
 
 /* Library build information: a static string naming the kernels compiled in. */
 const char* evo_build_info(void);
+
+/* The name of the last HIP error behind an EVO_ERR_DEVICE or EVO_ERR_OOM
+ * return of a device entry point on the calling thread ("hipSuccess" if
+ * none); a static string. Replaces the driver result a CudaError names
+ * (src/cuda_engine.rs:222-240). */
+const char* evo_device_error(void);
 
 #ifdef __cplusplus
 }

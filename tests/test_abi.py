@@ -28,11 +28,28 @@ def _declared() -> set[str]:
 def test_library_exports_every_declared_symbol():
     names = _declared()
     assert len(names) >= 13
+    N.lib()  # (binds the library to torch's HIP runtime before the raw handle)
     raw = C.CDLL(str(N.LIB_PATH))
     missing = [n for n in sorted(names) if not hasattr(raw, n)]
     assert not missing, f"declared in include/ but not exported: {missing}"
     # and the ctypes mirror binds exactly the declared set
     assert set(N.SIGNATURES) == names
+
+
+def test_one_hip_runtime_whatever_loads_first():
+    """torch carries its own HIP runtime under another file name; loading
+    this library before torch must not map a second runtime into the
+    process (launches there would not see torch's device)."""
+    import json
+    import subprocess
+    import sys
+
+    code = ("import json, evolution_simulator_gpu_amd._native as N; N.lib(); import torch; "
+            "print(json.dumps(sorted({l.split()[-1] for l in open('/proc/self/maps') if 'libamdhip64' in l})))")
+    out = subprocess.run([sys.executable, "-c", code], cwd=ROOT, capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr
+    mapped = json.loads(out.stdout.strip().splitlines()[-1])
+    assert len(mapped) == 1, mapped
 
 
 def test_struct_layouts_match_c():

@@ -15,6 +15,7 @@ import evolution_simulator_gpu_amd as E
 import oracle
 from evolution_simulator_gpu_amd import adjoint as A
 from evolution_simulator_gpu_amd import _native as N
+from evolution_simulator_gpu_amd import genome as G
 
 WORDS = [8, 9, 10, 3, 4, 5]  # muscle record words in gradient order
 
@@ -144,14 +145,27 @@ def test_adjoint_refuses_what_it_cannot_differentiate():
                                   16, 0, None) == N.EVO_ERR_INVALID  # workspace too small
     assert lib.evo_adjoint_forward_device(dummy, dummy, dummy, dummy, None, C.byref(p), C.byref(run), 60, dummy,
                                           1 << 40, None) == N.EVO_ERR_INVALID  # no claim counter
-    need = lib.evo_adjoint_workspace_bytes(4, p.steps, 8, 60)
-    # per lane (rounded up to a wave of 64): one checkpoint (node state,
-    # stores, rhythm offsets, reset times, ground bits), 480 substep start
-    # states (node state, stores) and tapes (10 words per bone, 2 per node),
-    # 60 step rhythm states and the lane's columns (offsets, reset times, six
+    # per lane (rounded up to a wave of 64), for the class of N nodes and M
+    # muscles the run selects: one checkpoint (node state, stores, rhythm
+    # offsets, reset times, ground bits), 480 substep start states (node
+    # state, stores) and tapes (10 words per bone, 2 per node), 60 step
+    # rhythm states and the lane's columns (offsets, reset times, six
     # gradient sums per muscle and one per node, eight constants per muscle)
-    assert need == (1 * (4 * 12 + 3 * 24 + 1) + 60 * 8 * (4 * 12 + 24 + 10 * 11 + 2 * 12) + 60 * 2 * 24
-                    + 8 * 24 + 12 + 8 * 24) * 64 * 4
+    def words(n, m):
+        return ((4 * n + 3 * m + 1) + 60 * 8 * ((4 * n + m) + (10 * (n - 1) + 2 * n)) + 60 * 2 * m
+                + 8 * m + n + 8 * m)
+
+    assert run.no_tendons and run.max_muscles <= 16
+    assert lib.evo_adjoint_workspace_bytes(C.byref(run), 4, p.steps, 60) == words(10, 16) * 64 * 4
+    run.no_tendons = 0
+    assert lib.evo_adjoint_workspace_bytes(C.byref(run), 4, p.steps, 60) == words(12, 24) * 64 * 4
+    run.max_muscles = 40
+    assert lib.evo_adjoint_workspace_bytes(C.byref(run), 65, p.steps, 60) == words(12, 64) * 128 * 4
+    assert lib.evo_adjoint_workspace_bytes(C.byref(run), 4, p.steps, 0) == 0
+    run.max_muscles = 65
+    assert lib.evo_adjoint_workspace_bytes(C.byref(run), 4, p.steps, 60) == 0
+    run.max_muscles, run.max_nodes = 40, 13
+    assert lib.evo_adjoint_workspace_bytes(C.byref(run), 4, p.steps, 60) == 0
 
 
 @pytest.mark.gpu
@@ -331,3 +345,58 @@ def test_gpu_adjoint_follows_sensor_resets(gpu):
             err = np.abs(g[c, :n] - want).max() / max(np.abs(want).max(), 1e-30)
             assert err < 1e-3, (terrain, c, err)
         assert sensing >= 6
+
+
+def _many_muscle_batch(world, count=64, seed=13):
+    """Bodies of at most 12 nodes and more than 24 muscles: synthetic ones
+    without tendons, or random genomes with tendons."""
+    if seed is None:
+        spec = E.synth_spec(2, count, world=world)
+        spec.max_nodes, spec.min_muscles = 12, 25
+        return E.synth(spec)
+    rng = np.random.default_rng(seed)
+    cs = [G.random_creature(rng, int(rng.integers(3, 13)), int(rng.integers(25, 61)), id=300 + i)
+          for i in range(count)]
+    return G.pack(cs, world)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", ["synthetic_flat", "tendons_rough", "synthetic_hard"])
+def test_gpu_many_muscle_adjoint_matches_the_dual_oracle(gpu, kind):
+    """The many-muscle class (at most 12 nodes, 25-64 muscles; muscles as
+    run-time loops in both halves), 1 s trials, checkpoints every 20 steps:
+    the pair's fitness equals the plain simulator's bit for bit, the
+    backward half from the forward's checkpoints equals the one that runs the
+    trial itself, and gradients agree with the FP32 dual oracle's to 1e-3 of
+    their largest component."""
+    import torch
+
+    worlds = {"synthetic_flat": dict(), "tendons_rough": dict(terrain=0.05),
+              "synthetic_hard": HARD_WORLDS["hard"]}
+    world = E.world(duration=1.0, substeps=8, **worlds[kind])
+    batch = _many_muscle_batch(world, seed=None if kind.startswith("synthetic") else 13)
+    assert batch.nodes().max() <= 12 and batch.muscle_counts().min() > 24
+    assert batch.tendon_free() == kind.startswith("synthetic")
+    p, run = E.run_for(batch, world)
+    pair = A.Pair(E.DeviceBatch(batch), p, run, seg=20)
+    pair.forward()
+    pair.backward()
+    torch.cuda.synchronize()
+    res = pair.db.results_numpy()
+    plain = E.simulate(batch, world)
+    assert np.array_equal(res.view(np.uint32), plain.view(np.uint32))
+    assert np.array_equal(pair.fitness.cpu().numpy().view(np.uint32), res["fitness"].view(np.uint32))
+    g = pair.gradient_numpy()
+    pair.backward(checkpointed=False)
+    torch.cuda.synchronize()
+    assert np.array_equal(pair.gradient_numpy(), g)
+    checked = 0
+    for c in range(0, batch.count, 6):
+        if (res["ground_hi"][c:c + 1].view(np.uint32)[0] & 32) != 0:
+            continue
+        _, want = oracle.full_gradient(batch, p, run, c, "f32")
+        assert np.all(g[c, want.size:] == 0.0)
+        err = np.abs(g[c, :want.size] - want).max() / max(np.abs(want).max(), 1e-30)
+        assert err < 1e-3, (kind, c, err)
+        checked += 1
+    assert checked >= 6
