@@ -3,7 +3,7 @@
 // world, or any world of the differentiated effects read from a scalar
 // register) a class for bodies of up to 10 nodes and 16 muscles without
 // tendons, one for up to 12 nodes and 24 muscles and one for up to 12 nodes
-// and 64 muscles, with and without tendons.
+// and 64 muscles, both with and without tendons.
 #include <hip/hip_runtime.h>
 
 #include "adjoint_kernel.hpp"
@@ -42,6 +42,7 @@ AdjointLauncher pick_world(const evo_run& run) {
         case Class::small:
             return &launch<Cfg<R, S, WORLD, kSmallNodes, kSmallMuscles, false, 64, true, false>>;
         case Class::mid:
+            if (run.no_tendons) { return &launch<Cfg<R, S, WORLD, kMidNodes, kMidMuscles, false, 64, true, false>>; }
             return &launch<Cfg<R, S, WORLD, kMidNodes, kMidMuscles, false, 64, true, true>>;
         case Class::many:
             if (run.no_tendons) {

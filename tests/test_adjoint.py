@@ -347,6 +347,37 @@ def test_gpu_adjoint_follows_sensor_resets(gpu):
         assert sensing >= 6
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("world", ["flat", "hard"])
+def test_gpu_mid_class_adjoint_matches_the_dual_oracle(gpu, world):
+    """Tendon-free bodies of up to 12 nodes and 24 constraints (the mid
+    class's tendon-free instance), 1 s trials: fitness as the plain
+    simulator's, gradients within 1e-3 of the FP32 dual oracle's largest
+    component."""
+    import torch
+
+    w = E.world(duration=1.0, substeps=8, **(HARD_WORLDS["hard"] if world == "hard" else {}))
+    batch = E.synth(E.synth_spec(1, 64, world=w))
+    p, run = E.run_for(batch, w)
+    assert run.no_tendons and run.max_nodes > 10 and run.max_muscles <= 24
+    pair = A.Pair(E.DeviceBatch(batch), p, run, seg=20)
+    pair.forward()
+    pair.backward()
+    torch.cuda.synchronize()
+    res = pair.db.results_numpy()
+    assert np.array_equal(res.view(np.uint32), E.simulate(batch, w).view(np.uint32))
+    g = pair.gradient_numpy()
+    checked = 0
+    for c in range(0, 64, 4):
+        if (res["ground_hi"][c:c + 1].view(np.uint32)[0] & 32) != 0:
+            continue
+        _, want = oracle.full_gradient(batch, p, run, c, "f32")
+        err = np.abs(g[c, :want.size] - want).max() / max(np.abs(want).max(), 1e-30)
+        assert err < 1e-3, (world, c, err)
+        checked += 1
+    assert checked >= 8
+
+
 def _many_muscle_batch(world, count=64, seed=13):
     """Bodies of at most 12 nodes and more than 24 muscles: synthetic ones
     without tendons, or random genomes with tendons."""
