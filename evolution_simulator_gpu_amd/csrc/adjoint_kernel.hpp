@@ -48,6 +48,14 @@ namespace adj {
 constexpr uint32_t kFlags = EVO_GROUND | EVO_TERRAIN | EVO_SLOPE | EVO_GAPS | EVO_HURDLES | EVO_QUAKE | EVO_ICE |
                             EVO_WIND | EVO_AIR;
 
+// Marks a tape value as unknown to the compiler where the reverse reads it,
+// so that the reverse recomputes what it derives from it instead of keeping
+// the forward pass's copies alive across the whole substep.
+__device__ __forceinline__ float fresh(float x) {
+    asm volatile("" : "+v"(x));
+    return x;
+}
+
 // Word offsets inside a tape record.
 template <class C>
 struct TapeAt {
@@ -91,16 +99,27 @@ struct TapeOut {
 
 // One substep's tape as the reverse reads it: the early half (drag and bone
 // vectors, read last) copied into the lane's LDS column `e`, the late half
-// (joints, ground, damping, read first) in registers.
+// (joints, ground, damping, read first) in registers. Where the node columns
+// and the whole early half would pass 40 KB of LDS (fewer than four blocks,
+// one wave per SIMD, on a CU), the bone vectors go to registers instead.
 template <class C>
 struct Tape {
+    static constexpr int kNodeLds = (C::MAXN + 1) * 64 * (16 + 8 + 16);
+    static constexpr bool kBonesInLds = kNodeLds + 4 * C::MAXB * 64 * 4 <= 40960;
+    static constexpr int kWords = (kBonesInLds ? 4 : 2) * C::MAXB;
     float* e;
     float jux[C::MAXB], juy[C::MAXB], jvx[C::MAXB], jvy[C::MAXB], jerr[C::MAXB];  // joint j before its push
     float gx[C::MAXN], gy[C::MAXN];    // node i before the ground
     float dlam[C::MAXB];               // joint j's damping multiplier
-    static constexpr int kWords = 4 * C::MAXB;
+    float bx[kBonesInLds ? 1 : C::MAXB], by[kBonesInLds ? 1 : C::MAXB];  // bone j's vector before its projection
     __device__ __forceinline__ float& dw(int j, int c) const { return e[(2 * j + c) * C::BLOCK]; }
-    __device__ __forceinline__ float& bd(int j, int c) const { return e[(2 * C::MAXB + 2 * j + c) * C::BLOCK]; }
+    __device__ __forceinline__ float bd(int j, int c) const {
+        if constexpr (kBonesInLds) {
+            return e[(2 * C::MAXB + 2 * j + c) * C::BLOCK];
+        } else {
+            return fresh(c == 0 ? bx[j] : by[j]);
+        }
+    }
 
     // Reads the wave's record `rec` (word w of the lane's at rec[w * 64 +
     // lane]) for the wave's bones and nodes: the late half into registers,
@@ -121,8 +140,13 @@ struct Tape {
             const int j = jc;
             dma(T::drag + 2 * j);
             dma(T::drag + 2 * j + 1);
-            dma(T::bone + 2 * j);
-            dma(T::bone + 2 * j + 1);
+            if constexpr (kBonesInLds) {
+                dma(T::bone + 2 * j);
+                dma(T::bone + 2 * j + 1);
+            } else {
+                bx[j] = at(T::bone + 2 * j);
+                by[j] = at(T::bone + 2 * j + 1);
+            }
         });
         items<C, C::MAXB, 1, 2>(wb.b, [&](auto jc) {
             const int j = jc;
@@ -193,13 +217,6 @@ __device__ __forceinline__ void items_down(uint32_t bound, F&& f) {
     down_from<N - 1, START>(bound, f);
 }
 
-// Marks a tape value as unknown to the compiler where the reverse reads it,
-// so that the reverse recomputes what it derives from it instead of keeping
-// the forward pass's copies alive across the whole substep.
-__device__ __forceinline__ float fresh(float x) {
-    asm volatile("" : "+v"(x));
-    return x;
-}
 
 // Marks every constant of the body as changed, at the top of each substep:
 // what the substep derives from them is then computed where it is used, not
