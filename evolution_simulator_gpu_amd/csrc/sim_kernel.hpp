@@ -118,6 +118,25 @@ __device__ __forceinline__ void groups_from(uint32_t bound, F& f) {
         }
     }
 }
+// Items START .. bound-1 of at most N as a run-time loop of groups of four
+// straight-line copies, so that the four items' loads (constants in scratch,
+// node rows in LDS) go out together instead of each waiting for the last.
+// Like the unrolled classes' groups, a group may run items past the wave's
+// bound (below N): those are spare items of every lane.
+template <int N, int START, class F>
+__device__ __forceinline__ void loop_groups(uint32_t bound, F& f) {
+    constexpr uint32_t n = static_cast<uint32_t>(N);
+    const uint32_t up = (bound + 3u) & ~3u;
+    const uint32_t end = up < n ? up : n;
+    uint32_t i = START;
+    for (; i + 4u <= end; i += 4u) {
+        f(static_cast<int>(i));
+        f(static_cast<int>(i + 1u));
+        f(static_cast<int>(i + 2u));
+        f(static_cast<int>(i + 3u));
+    }
+    for (; i < end; i++) { f(static_cast<int>(i)); }
+}
 // Items START .. bound-1 of at most N, where `bound` is the same for the whole
 // wave. An unrolled class runs them as straight-line copies in groups of G
 // behind one scalar branch per group; a looped class runs a scalar loop.
@@ -126,11 +145,7 @@ __device__ __forceinline__ void items(uint32_t bound, F&& f) {
     if constexpr (C::unrolled) {
         groups_from<START, N, G>(bound, f);
     } else {
-        const uint32_t end = bound < static_cast<uint32_t>(N) ? bound : static_cast<uint32_t>(N);
-        // four items per trip: their loads of the constants in scratch go out
-        // together instead of each waiting for the last (1.7x faster)
-#pragma unroll 4
-        for (uint32_t i = START; i < end; i++) { f(static_cast<int>(i)); }
+        loop_groups<N, START>(bound, f);
     }
 }
 // Items 0 .. N-1 regardless of counts (set-up of a new creature).
@@ -148,9 +163,7 @@ __device__ __forceinline__ void mitems(uint32_t bound, F&& f) {
     if constexpr (C::munrolled) {
         groups_from<START, N, G>(bound, f);
     } else {
-        const uint32_t end = bound < static_cast<uint32_t>(N) ? bound : static_cast<uint32_t>(N);
-#pragma unroll 4
-        for (uint32_t i = START; i < end; i++) { f(static_cast<int>(i)); }
+        loop_groups<N, START>(bound, f);
     }
 }
 template <class C, int N, class F>
