@@ -200,7 +200,16 @@ struct ACols {
         asm volatile("" : "+v"(base));
         __builtin_amdgcn_global_atomic_fadd_f32(base + (kSum + q) * 64, v);
     }
-    __device__ __forceinline__ void add_muscle(int k, int f, float v) const { add(6 * k + f, v); }
+    __device__ __forceinline__ void add_muscle(int k, int f, float v) const {
+        if constexpr (C::munrolled) {
+            add(6 * k + f, v);
+        } else {
+            // (a run-time muscle loop forms the address each trip anyway; the
+            // marker would keep the next muscles' loads from going out early)
+            using gfloat = __attribute__((address_space(1))) float;
+            __builtin_amdgcn_global_atomic_fadd_f32((gfloat*)(cb + lane) + (kSum + 6 * k + f) * 64, v);
+        }
+    }
     __device__ __forceinline__ void add_fric(int i, float v) const { add(6 * C::MAXM + i, v); }
 };
 
@@ -696,7 +705,7 @@ __device__ __forceinline__ void muscles_back(const Body<C>& b, const Cols<C>& s,
     mitems<C, C::MAXM, 0, 1>(wb.m, [&](auto kc) {
         const int k = kc;
         uint32_t nodes = b.mnodes[k];
-        opaque<C>(nodes);
+        mopaque<C>(nodes);
         const uint32_t a0 = nodes & 255u, a1 = (nodes >> 8u) & 255u, b0 = (nodes >> 16u) & 255u, b1 = nodes >> 24u;
         const float aa = b.maa[k], ab = b.mab[k];
         const float ra = 1.0f - aa, rb = 1.0f - ab;

@@ -181,6 +181,13 @@ template <class C>
 __device__ __forceinline__ void opaque(uint32_t& x) {
     if constexpr (C::unrolled) { asm volatile("" : "+v"(x)); }
 }
+// The same for a muscle's words: only where muscles are unrolled. In a
+// run-time muscle loop the words are loaded afresh each trip anyway, and the
+// marker would keep the next muscles' loads from being issued early.
+template <class C>
+__device__ __forceinline__ void mopaque(uint32_t& x) {
+    if constexpr (C::munrolled) { asm volatile("" : "+v"(x)); }
+}
 
 // ---------------------------------------------------------------- math
 
@@ -473,7 +480,7 @@ __device__ __forceinline__ void muscle_forces(Body<C>& b, const Cols<C>& s, floa
     mitems<C, C::MAXM, 0, 2>(wb.m, [&](auto kc) {
         const int k = kc;
         uint32_t nodes = b.mnodes[k];
-        opaque<C>(nodes);
+        mopaque<C>(nodes);
         const uint32_t a0 = nodes & 255u, a1 = (nodes >> 8u) & 255u, b0 = (nodes >> 16u) & 255u, b1 = nodes >> 24u;
         const float aa = b.maa[k], ab = b.mab[k];
         const float4 qa0 = s.p(a0), qa1 = s.p(a1), qb0 = s.p(b0), qb1 = s.p(b1);
