@@ -424,7 +424,7 @@ __device__ __forceinline__ void damp_back(const Body<C>& b, const Cols<C>& s, co
     items_down<C::MAXB, 1>(wb.b, [&](auto jc) {
         const int j = jc;
         uint32_t topo = b.btopo[j];
-        opaque<C>(topo);
+        opaque<C>(topo, wb);
         const Joint k = joint<C>(b, s, j, topo, false);
         const float wgt = joint_weight(k);
         const bool on = (topo & kHasJoint) != 0u && wgt > 0.0f;
@@ -549,7 +549,7 @@ __device__ __forceinline__ void joints_back(const Body<C>& b, const ACols<C>& a,
     items_down<C::MAXB, 1>(wb.b, [&](auto jc) {
         const int j = jc;
         uint32_t topo = b.btopo[j];
-        opaque<C>(topo);
+        opaque<C>(topo, wb);
         const Joint k = joint_from<C>(b, j, topo, fresh(t.jux[j]), fresh(t.juy[j]), fresh(t.jvx[j]), fresh(t.jvy[j]));
         const float error = fresh(t.jerr[j]);
         const float wgt = joint_weight(k);
@@ -572,7 +572,7 @@ __device__ __forceinline__ void bones_back(const Body<C>& b, const ACols<C>& a, 
     items_down<C::MAXB, 0>(wb.b, [&](auto jc) {
         const int j = jc;
         uint32_t topo = b.btopo[j];
-        opaque<C>(topo);
+        opaque<C>(topo, wb);
         const uint32_t i0 = topo & 63u;
         const float dx = t.bd(j, 0), dy = t.bd(j, 1);
         const float raw = sqrtf(__builtin_fmaf(dx, dx, dy * dy));
@@ -632,7 +632,7 @@ __device__ __forceinline__ void drag_back(const Body<C>& b, const ACols<C>& a, c
     items_down<C::MAXB, 0>(wb.b, [&](auto jc) {
         const int j = jc;
         uint32_t topo = b.btopo[j];
-        opaque<C>(topo);
+        opaque<C>(topo, wb);
         const uint32_t i0 = topo & 63u;
         const float wx = t.dw(j, 0), wy = t.dw(j, 1);
         const float speed = sqrtf(__builtin_fmaf(wx, wx, wy * wy));
@@ -705,7 +705,7 @@ __device__ __forceinline__ void muscles_back(const Body<C>& b, const Cols<C>& s,
     mitems<C, C::MAXM, 0, 1>(wb.m, [&](auto kc) {
         const int k = kc;
         uint32_t nodes = b.mnodes[k];
-        mopaque<C>(nodes);
+        mopaque<C>(nodes, wb);
         const uint32_t a0 = nodes & 255u, a1 = (nodes >> 8u) & 255u, b0 = (nodes >> 16u) & 255u, b1 = nodes >> 24u;
         const float aa = b.maa[k], ab = b.mab[k];
         const float ra = 1.0f - aa, rb = 1.0f - ab;
@@ -1170,7 +1170,7 @@ __global__ void __launch_bounds__(64) adjoint_kernel(const uint32_t* __restrict_
     if (!in_launch || !fits || (code & 32u) != 0u) { steps_l = 0u; }
     const uint32_t S = wave_max(steps_l);
     const bool sensing = wave_any(sensor && steps_l > 0u);
-    const Bounds wb{wave_max(b.nodes), wave_max(b.bones), wave_max(b.muscles)};
+    const Bounds wb{wave_max(b.nodes), wave_max(b.bones), wave_max(b.muscles), 0u};
     const Work<C> work{ckpt, slots, tapes, steps_ws, stride, blockIdx.x, lane};
     put_muscles<C>(a, b);
     uint32_t ground_bits = 0u;
@@ -1184,7 +1184,7 @@ __global__ void __launch_bounds__(64) adjoint_kernel(const uint32_t* __restrict_
         demands<C>(b, a, mus, t, sensing, wb);
         for (uint32_t sub = 0; sub < C::substeps; sub++) {
             refresh<C>(b);
-            substep<C>(b, s, p, flags, rec_h, wb);
+            substep<C>(b, s, p, flags, rec_h, Bounds{wb.n, wb.b, wb.m, sub});
         }
         if (sensing) { sensors<C>(b, s, a, mus, p, flags, st, ground_bits, wb); }
         if (st + 1u == steps_l) { fit = com_x<C>(b, s, wb); }
@@ -1222,7 +1222,7 @@ __global__ void __launch_bounds__(64) adjoint_kernel(const uint32_t* __restrict_
                 const uint32_t slot = (st - s0) * C::substeps + sub;
                 work.put_slot(slot, b, s, wb);
                 const TapeOut<C> out{work.tape(slot), lane};
-                substep<C, const TapeOut<C>>(b, s, p, flags, rec_h, wb, &out);
+                substep<C, const TapeOut<C>>(b, s, p, flags, rec_h, Bounds{wb.n, wb.b, wb.m, sub}, &out);
             }
             if (sensing) { sensors<C>(b, s, a, mus, p, flags, st, ground_bits, wb); }
         }
@@ -1256,7 +1256,7 @@ __global__ void __launch_bounds__(64) adjoint_kernel(const uint32_t* __restrict_
                 });
                 if (live) {
                     substep_back<C>(
-                        b, s, a, tape, g, p, flags, rec_h, wb,
+                        b, s, a, tape, g, p, flags, rec_h, Bounds{wb.n, wb.b, wb.m, sub},
                         [&]() {
                             items<C, C::MAXN, 0, 4>(wb.n, [&](auto ic) {
                                 const int i = ic;

@@ -175,18 +175,27 @@ __device__ __forceinline__ void mall_items(F&& f) {
     }
 }
 
+// The wave's largest node, bone and muscle counts: the scalar loop bounds.
+struct Bounds {
+    uint32_t n, b, m;
+    // a value that changes every substep (see opaque)
+    uint32_t salt;
+};
+
 // Marks a register as changed, so the compiler recomputes what depends on it
-// instead of keeping a copy live across the substep loop.
+// instead of keeping a copy live across the substep loop. The marker reads
+// the substep's salt, so it cannot be moved out of the loop; it has no side
+// effects, so loads and stores of neighbouring items can still be scheduled
+// across it (a volatile marker would order every memory access around it).
 template <class C>
-__device__ __forceinline__ void opaque(uint32_t& x) {
-    if constexpr (C::unrolled) { asm volatile("" : "+v"(x)); }
+__device__ __forceinline__ void opaque(uint32_t& x, const Bounds& wb) {
+    if constexpr (C::unrolled) { asm("" : "+v"(x) : "s"(wb.salt)); }
 }
 // The same for a muscle's words: only where muscles are unrolled. In a
-// run-time muscle loop the words are loaded afresh each trip anyway, and the
-// marker would keep the next muscles' loads from being issued early.
+// run-time muscle loop the words are loaded afresh each trip anyway.
 template <class C>
-__device__ __forceinline__ void mopaque(uint32_t& x) {
-    if constexpr (C::munrolled) { asm volatile("" : "+v"(x)); }
+__device__ __forceinline__ void mopaque(uint32_t& x, const Bounds& wb) {
+    if constexpr (C::munrolled) { asm("" : "+v"(x) : "s"(wb.salt)); }
 }
 
 // ---------------------------------------------------------------- math
@@ -306,10 +315,6 @@ struct Cols {
     __device__ __forceinline__ float& o(uint32_t k) const { return off[k * (C::munrolled ? C::BLOCK : 1u)]; }
 };
 
-// The wave's largest node, bone and muscle counts: the scalar loop bounds.
-struct Bounds {
-    uint32_t n, b, m;
-};
 
 // The trial so far (shaders/creature.cu:783-826).
 struct Trial {
@@ -480,7 +485,7 @@ __device__ __forceinline__ void muscle_forces(Body<C>& b, const Cols<C>& s, floa
     mitems<C, C::MAXM, 0, 2>(wb.m, [&](auto kc) {
         const int k = kc;
         uint32_t nodes = b.mnodes[k];
-        mopaque<C>(nodes);
+        mopaque<C>(nodes, wb);
         const uint32_t a0 = nodes & 255u, a1 = (nodes >> 8u) & 255u, b0 = (nodes >> 16u) & 255u, b1 = nodes >> 24u;
         const float aa = b.maa[k], ab = b.mab[k];
         const float4 qa0 = s.p(a0), qa1 = s.p(a1), qb0 = s.p(b0), qb1 = s.p(b1);
@@ -553,7 +558,7 @@ __device__ __forceinline__ void bone_drag(const Body<C>& b, const Cols<C>& s, co
     items<C, C::MAXB, 0, 2>(wb.b, [&](auto jc) {
         const int j = jc;
         uint32_t topo = b.btopo[j];
-        opaque<C>(topo);
+        opaque<C>(topo, wb);
         const uint32_t i0 = topo & 63u;
         float4& q0r = s.p(i0);
         float4& q1r = s.p(j + 1);
@@ -595,7 +600,7 @@ __device__ __forceinline__ void solve_bones(const Body<C>& b, const Cols<C>& s, 
     items<C, C::MAXB, 0, 1>(wb.b, [&](auto jc) {
         const int j = jc;
         uint32_t topo = b.btopo[j];
-        opaque<C>(topo);
+        opaque<C>(topo, wb);
         const uint32_t i0 = topo & 63u;
         float4& q0r = s.p(i0);
         float4& q1r = s.p(j + 1);
@@ -622,7 +627,7 @@ __device__ __forceinline__ void solve_joints(const Body<C>& b, const Cols<C>& s,
     items<C, C::MAXB, 1, 1>(wb.b, [&](auto jc) {
         const int j = jc;
         uint32_t topo = b.btopo[j];
-        opaque<C>(topo);
+        opaque<C>(topo, wb);
         const Joint k = joint<C>(b, s, j, topo, true);
         float error = 0.0f;
         if (k.angle < k.lo) { error = k.angle - k.lo; }
@@ -642,7 +647,7 @@ __device__ __forceinline__ void damp_joints(const Body<C>& b, const Cols<C>& s, 
     items<C, C::MAXB, 1, 1>(wb.b, [&](auto jc) {
         const int j = jc;
         uint32_t topo = b.btopo[j];
-        opaque<C>(topo);
+        opaque<C>(topo, wb);
         const Joint k = joint<C>(b, s, j, topo, false);
         const float4 vt = s.p(k.tip), vf = s.p(k.far_), vh = s.p(k.hub);
         float rate = k.gtx * vt.z;
@@ -1036,7 +1041,10 @@ __device__ __forceinline__ void run_step(Body<C>& b, Trial& tr, const Cols<C>& s
         });
     }
     muscle_demands<C>(b, s, mus, t_now, tr.limp, wb);
-    for (uint32_t sub = 0u; sub < C::substeps; sub++) { substep<C>(b, s, p, flags, rec_h, wb); }
+    for (uint32_t sub = 0u; sub < C::substeps; sub++) {
+        const Bounds ws{wb.n, wb.b, wb.m, sub};
+        substep<C>(b, s, p, flags, rec_h, ws);
+    }
     if constexpr (C::record) {
         const float inv_sub = 1.0f / static_cast<float>(C::substeps);
         all_items<C, C::MAXN>([&](auto ic) {
@@ -1289,7 +1297,7 @@ __global__ void __launch_bounds__(C::BLOCK) advance_kernel(const uint32_t* __res
     uint32_t win_cur = 0u, win_end = 0u;
     bool claimed_all = false;
     float2* my_frames = nullptr;
-    for (;;) {
+    for (uint32_t round = 0u;; round++) {
         uint64_t need = ~__ballot(tr.live);
         uint32_t mine_at = 0u;
         bool fresh = false;
@@ -1345,13 +1353,13 @@ __global__ void __launch_bounds__(C::BLOCK) advance_kernel(const uint32_t* __res
         }
         if (__ballot(tr.live) == 0ull) { break; }
         const Bounds wb{wave_max(tr.live ? b.nodes : 0u), wave_max(tr.live ? b.bones : 0u),
-                        wave_max(tr.live ? b.muscles : 0u)};
+                        wave_max(tr.live ? b.muscles : 0u), round};
         if (tr.live) {
             if constexpr (C::world == kTableWorld) {
                 // the lane's world: its parameters are read where they are
                 // used (L1-resident table), not held in registers
                 uint32_t w = tr.world;
-                opaque<C>(w);
+                opaque<C>(w, wb);
                 const evo_params& pw = worlds.params[w];
                 const float rh = 0.5f * pw.muscle_recovery * C::h;
                 run_step<C>(b, tr, s, muscles, results, pw, worlds.flags[w], rh, my_frames, wb, ck);
