@@ -1120,14 +1120,18 @@ __global__ void __launch_bounds__(64) adjoint_kernel(const uint32_t* __restrict_
     __shared__ float4 s_adj[(C::MAXN + 1) * 64];
     __shared__ float s_tape[Tape<C>::kWords * 64];
     const uint32_t lane = threadIdx.x;
-    const uint32_t idx = blockIdx.x * 64u + lane;
+    // blocks take the launch's waves from the last: a packed batch ascends
+    // in body size, so the heaviest waves start first and the last blocks to
+    // start (when the grid is larger than what fits at once) are the lightest
+    const uint32_t wave = gridDim.x - 1u - blockIdx.x;
+    const uint32_t idx = wave * 64u + lane;
     const bool in_launch = idx < p.count;
     const uint32_t c = p.base + (in_launch ? idx : p.count - 1u);
     Cols<C> s;
     s.pv = s_pv + lane;
     s.fp = s_fp + lane;
     // the block's device-memory columns: offsets, reset times, gradient sums
-    float* const cb = rhythm + static_cast<uint64_t>(blockIdx.x) * (Words<C::MAXN, C::MAXM>::columns * 64u);
+    float* const cb = rhythm + static_cast<uint64_t>(wave) * (Words<C::MAXN, C::MAXM>::columns * 64u);
     // (the simulator's set-up clears the offsets through this pointer; with
     // muscles as run-time loops its offsets have a stride of one, so it
     // clears a private array the adjoint does not read)
@@ -1171,7 +1175,7 @@ __global__ void __launch_bounds__(64) adjoint_kernel(const uint32_t* __restrict_
     const uint32_t S = wave_max(steps_l);
     const bool sensing = wave_any(sensor && steps_l > 0u);
     const Bounds wb{wave_max(b.nodes), wave_max(b.bones), wave_max(b.muscles), 0u};
-    const Work<C> work{ckpt, slots, tapes, steps_ws, stride, blockIdx.x, lane};
+    const Work<C> work{ckpt, slots, tapes, steps_ws, stride, wave, lane};
     put_muscles<C>(a, b);
     uint32_t ground_bits = 0u;
     float fit = results[c].fitness;
