@@ -386,12 +386,12 @@ def full(args, torch):
         torch.cuda.synchronize()
         pair_fwd_ms, pair_bwd_ms = e[0].elapsed_time(e[1]), e[1].elapsed_time(e[2])
         note(f"pairs on {chosen.size} elites: forward {pair_fwd_ms:.0f} ms, backward {pair_bwd_ms:.0f} ms")
-        grads = [pair.gradient_numpy() for pair in pairs]
-        # gradient-directed mutation of every elite with a finite gradient
+        # gradient-directed mutation of every elite with a finite gradient,
+        # on the device, in place on the pairs' copies of the elites
         t2 = time.perf_counter()
-        finite = [np.isfinite(g).all(axis=1) for g in grads]
-        mutated = [A.ascend_batch(eb, np.where(f[:, None], g, 0.0), rate=0.02)
-                   for eb, g, f in zip(subsets, grads, finite)]
+        for pair in pairs:
+            A.ascend_device(pair.db, pair.grads, rate=0.02)
+        torch.cuda.synchronize()
         mutate_s = time.perf_counter() - t2
         # replay capture of the fittest
         e = [ev() for _ in range(2)]
@@ -409,6 +409,18 @@ def full(args, torch):
     same = True
     rel32 = []
     rng = np.random.default_rng(12)
+    grads = [pair.gradient_numpy() for pair in pairs]
+    finite = [np.isfinite(g).all(axis=1) for g in grads]
+    # the device mutation of the last generation against the host one
+    mutated_same = True
+    mutated_valid = True
+    for pair, grad, fin in zip(pairs, grads, finite):
+        want = A.ascend_batch(pair.db.batch, np.where(fin[:, None], grad, 0.0), rate=0.02)
+        got = E.Batch(pair.db.records.cpu().numpy().view(np.uint32), pair.db.muscles.cpu().numpy(),
+                      pair.db.batch.heads, pair.db.batch.slots)
+        mutated_same = mutated_same and bool(np.array_equal(got.records, want.records)) and bool(
+            np.array_equal(got.muscles.view(np.uint32), want.muscles.view(np.uint32)))
+        mutated_valid = mutated_valid and got.validate() is None
     for g, pair, grad, fin in zip(groups, pairs, grads, finite):
         pres = pair.db.results_numpy()
         elite_substeps += int((pres["ground_hi"].view(np.uint32) >> 16).astype(np.int64).sum()) * world.substeps
@@ -457,7 +469,8 @@ def full(args, torch):
             "elite_fitness_bit_exact": same,
             "elite_gradients_beyond_float_range": int(sum((~f).sum() for f in finite)),
             "mutation_s": mutate_s,
-            "mutated_records_valid": all(m.validate() is None for m in mutated),
+            "mutated_records_valid": mutated_valid,
+            "device_mutation_equals_host": mutated_same,
             "replays": replays,
             "replay_ms": replay_ms,
             "elite_gradients_vs_fp32_dual_oracle": {"sample": int(rel32.size),

@@ -250,12 +250,20 @@ def ascend_batch(batch: Batch, grads: np.ndarray, rate: float = 0.02) -> Batch:
         "strength": g_cap * 200.0 + g_invcap * (-1.0 / (120.0 * strength ** 2)),
         "friction": gf,
     }
+    def owner_max(owner, v):
+        # per creature, the largest of its items' values (items are grouped
+        # by creature in ascending order)
+        out = np.zeros(batch.count)
+        if owner.size:
+            starts = np.flatnonzero(np.r_[True, owner[1:] != owner[:-1]])
+            out[owner[starts]] = np.maximum.reduceat(v, starts)
+        return np.maximum(out, 0.0)
+
     for key in GENES:
         owner = nc if key == "friction" else mc
         scale = np.maximum(np.abs(genes[key]), 1e-3)
         d = gg[key] * scale
-        norm = np.zeros(batch.count)
-        np.maximum.at(norm, owner, np.abs(d))
+        norm = owner_max(owner, np.abs(d))
         nz = norm[owner] > 0.0
         step = np.where(nz, rate * scale * d / np.where(nz, norm[owner], 1.0), 0.0)
         genes[key] = np.where(nz, genes[key] + step, genes[key])
@@ -279,6 +287,20 @@ def ascend_batch(batch: Batch, grads: np.ndarray, rate: float = 0.02) -> Batch:
     om[mrow, 5] = f32(1.0) / (f32(120.0) * stg)
     out.records[nword] = genes["friction"].astype(f32).view(np.uint32)
     return out
+
+
+def ascend_device(db: DeviceBatch, grads, rate: float = 0.02, stream=None) -> None:
+    """ascend_batch on the device (evo_ascend_device), in place on the
+    device-resident batch `db`: grads is a device tensor of STRIDE floats per
+    creature (Pair.grads); rows with a component beyond float range move
+    nothing, as the host mutation given those rows zeroed."""
+    import torch
+
+    s = stream if stream is not None else torch.cuda.current_stream(db.records.device)
+    rc = N.lib().evo_ascend_device(db.records.data_ptr(), db.muscles.data_ptr(), db.heads.data_ptr(),
+                                   grads.data_ptr(), db.batch.count, STRIDE, float(rate), C.c_void_p(s.cuda_stream))
+    if rc != N.EVO_OK:
+        raise RuntimeError(f"evo_ascend_device failed: {rc} ({N.lib().evo_device_error().decode()})")
 
 
 def gene_direction(genes: dict, rng) -> dict:

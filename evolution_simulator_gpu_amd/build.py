@@ -39,7 +39,7 @@ HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", f"-I{I
 
 RATES_SUBSTEPS = [(60, 8), (60, 16), (120, 8), (120, 16)]
 HOST_SOURCES = ["world.cpp", "synth.cpp", "batch.cpp", "half.cpp", "islands.cpp", "pack.cpp", "replay.cpp"]
-HIP_SOURCES = ["advance.hip", "describe.hip", "adjoint.hip", "engine.hip"]
+HIP_SOURCES = ["advance.hip", "describe.hip", "adjoint.hip", "ascend.hip", "engine.hip"]
 
 
 def _hipcc() -> str:

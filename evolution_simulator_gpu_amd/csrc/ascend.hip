@@ -1,0 +1,159 @@
+// Gradient-directed mutation of a device-resident batch: every creature's
+// genes take a step along its fitness gradient (the adjoint's output), one
+// thread per creature. The genes behind the records are those the generator
+// and the reference's genome builder derive them from (src/physics2.rs:233-262,
+// src/kernel.rs:376-408): per muscle the rhythm period and phase, the
+// contracted and extended length and the strength; per node the friction.
+//
+// The arithmetic is the host mutation's (adjoint.ascend_batch) operation for
+// operation: genes and steps in double precision, the rebuilt records in
+// single precision, no contraction, so the device result equals it bit for
+// bit.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../include/evo_engine.h"
+#include "launch.hpp"
+
+namespace {
+
+constexpr double kDutySpeed = 24.0;  // the rhythm's speed limit behind the amplitude cap (m/s)
+constexpr double kPi = 3.141592653589793;
+
+__device__ __forceinline__ double dmin(double a, double b) { return (a < b || a != a) ? a : b; }
+__device__ __forceinline__ double dmax(double a, double b) { return (a > b || a != a) ? a : b; }
+__device__ __forceinline__ float fmn(float a, float b) { return (a < b || a != a) ? a : b; }
+__device__ __forceinline__ float fmx(float a, float b) { return (a > b || a != a) ? a : b; }
+
+// numpy's floating remainder: the sign follows the divisor.
+__device__ __forceinline__ double pymod(double a, double b) {
+    double m = fmod(a, b);
+    if (m != 0.0) {
+        if ((b < 0.0) != (m < 0.0)) { m += b; }
+    } else {
+        m = copysign(0.0, b);
+    }
+    return m;
+}
+
+// A muscle's genes and the gradient with respect to them.
+struct MuscleGenes {
+    double period, duty, phase, extended, contracted, strength;
+    double g[5];  // period, phase, contracted, extended, strength
+};
+
+__device__ __forceinline__ MuscleGenes muscle_genes(const float* m, const double* g6) {
+    MuscleGenes x;
+    x.period = 1.0 / static_cast<double>(m[9]);
+    x.duty = static_cast<double>(m[11]);
+    x.phase = static_cast<double>(m[10]);
+    x.extended = 1.0 / (8.0 * static_cast<double>(m[3]));
+    x.contracted = x.extended - static_cast<double>(m[8]);
+    x.strength = static_cast<double>(m[4]) / 200.0;
+    const double cap_amp = ((2.0 * kDutySpeed) * x.period) * dmin(x.duty, 1.0 - x.duty) / kPi;
+    const bool capped = (x.extended - x.contracted) > cap_amp;
+    const double g_amp = g6[0], g_invp = g6[1], g_phase = g6[2], g_hill = g6[3], g_cap = g6[4], g_invcap = g6[5];
+    x.g[0] = g_invp * (-1.0 / (x.period * x.period)) + (capped ? g_amp * cap_amp / x.period : 0.0);
+    x.g[1] = g_phase;
+    x.g[2] = capped ? 0.0 : -g_amp;
+    x.g[3] = (capped ? 0.0 : g_amp) +
+             (x.extended > 0.05 ? g_hill * (-1.0 / (8.0 * (x.extended * x.extended))) : 0.0);
+    x.g[4] = g_cap * 200.0 + g_invcap * (-1.0 / (120.0 * (x.strength * x.strength)));
+    return x;
+}
+
+__device__ __forceinline__ double gene(const MuscleGenes& x, int k) {
+    return k == 0 ? x.period : k == 1 ? x.phase : k == 2 ? x.contracted : k == 3 ? x.extended : x.strength;
+}
+
+__global__ void __launch_bounds__(256) ascend_kernel(uint32_t* __restrict__ records, float* __restrict__ muscles,
+                                                    const uint4* __restrict__ heads,
+                                                    const float* __restrict__ grads, uint32_t count,
+                                                    uint32_t stride, double rate) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= count) { return; }
+    const uint4 h0 = heads[2u * c], h1 = heads[2u * c + 1u];
+    const uint32_t n = h0.x & 255u, m = h0.y;
+    uint32_t* rec = records + h0.w;
+    float* mus = muscles + h1.x;
+    const float* gr = grads + static_cast<uint64_t>(c) * stride;
+    // a gradient row with a component beyond float range moves nothing
+    bool finite = true;
+    for (uint32_t q = 0; q < 6u * m + n; q++) { finite = finite && isfinite(gr[q]); }
+    const auto grad = [&](uint32_t q) { return finite ? static_cast<double>(gr[q]) : 0.0; };
+
+    // the largest scaled step of each gene over the creature's items
+    double norm[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (uint32_t k = 0; k < m; k++) {
+        double g6[6];
+        for (int f = 0; f < 6; f++) { g6[f] = grad(6u * k + f); }
+        const MuscleGenes x = muscle_genes(mus + k * EVO_MUSCLE_WORDS, g6);
+        for (int j = 0; j < 5; j++) {
+            const double scale = dmax(fabs(gene(x, j)), 1e-3);
+            norm[j] = dmax(norm[j], fabs(x.g[j] * scale));
+        }
+    }
+    for (uint32_t i = 0; i < n; i++) {
+        const double fr = static_cast<double>(__uint_as_float(rec[i * EVO_NODE_WORDS + 2u]));
+        const double scale = dmax(fabs(fr), 1e-3);
+        norm[5] = dmax(norm[5], fabs(grad(6u * m + i) * scale));
+    }
+
+    for (uint32_t k = 0; k < m; k++) {
+        float* mk = mus + k * EVO_MUSCLE_WORDS;
+        double g6[6];
+        for (int f = 0; f < 6; f++) { g6[f] = grad(6u * k + f); }
+        const MuscleGenes x = muscle_genes(mk, g6);
+        double v[5];
+        for (int j = 0; j < 5; j++) {
+            v[j] = gene(x, j);
+            const double scale = dmax(fabs(v[j]), 1e-3);
+            const double d = x.g[j] * scale;
+            if (norm[j] > 0.0) { v[j] = v[j] + rate * scale * d / norm[j]; }
+        }
+        double period = dmin(dmax(v[0], 0.5), 2.0);
+        double phase = pymod(v[1], 1.0);
+        double extended = v[3];
+        double contracted = dmin(v[2], extended);
+        double strength = dmin(dmax(v[4], 0.1), 1.0);
+        // the records, in single precision
+        const float per = static_cast<float>(period), dty = static_cast<float>(x.duty);
+        const float lng = static_cast<float>(extended), sht = static_cast<float>(contracted);
+        const float stg = static_cast<float>(strength);
+        const float cap = ((2.0f * static_cast<float>(kDutySpeed)) * per * fmn(dty, 1.0f - dty)) /
+                          static_cast<float>(kPi);
+        mk[8] = fmn(lng - sht, cap);
+        mk[9] = 1.0f / per;
+        mk[10] = static_cast<float>(phase);
+        mk[3] = 1.0f / (8.0f * fmx(lng, 0.05f));
+        mk[4] = 200.0f * stg;
+        mk[5] = 1.0f / (120.0f * stg);
+    }
+    for (uint32_t i = 0; i < n; i++) {
+        uint32_t& w = rec[i * EVO_NODE_WORDS + 2u];
+        double fr = static_cast<double>(__uint_as_float(w));
+        const double scale = dmax(fabs(fr), 1e-3);
+        const double d = grad(6u * m + i) * scale;
+        if (norm[5] > 0.0) { fr = fr + rate * scale * d / norm[5]; }
+        fr = dmin(dmax(fr, 0.2), 1.0);
+        w = __float_as_uint(static_cast<float>(fr));
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int evo_ascend_device(uint32_t* d_records, float* d_muscles, const uint32_t* d_heads, const float* d_grads,
+                      uint32_t count, uint32_t grad_stride, double rate, void* stream) {
+    if (!d_records || !d_muscles || !d_heads || !d_grads || grad_stride == 0u) { return EVO_ERR_INVALID; }
+    if ((reinterpret_cast<uintptr_t>(d_heads) & 15u) != 0u) { return EVO_ERR_INVALID; }
+    if (count == 0u) { return EVO_OK; }
+    const uint32_t blocks = (count + 255u) / 256u;
+    hipLaunchKernelGGL(ascend_kernel, dim3(blocks), dim3(256), 0, reinterpret_cast<hipStream_t>(stream), d_records,
+                       d_muscles, reinterpret_cast<const uint4*>(d_heads), d_grads, count, grad_stride, rate);
+    return evo::dev::device_code(hipGetLastError());
+}
+
+}  // extern "C"

@@ -287,6 +287,22 @@ int evo_adjoint_device(const uint32_t* d_records, const float* d_muscles, const 
                        float* d_grads, float* d_fitness, void* d_workspace, uint64_t workspace_bytes,
                        uint32_t checkpointed, void* stream);
 
+/* Gradient-directed mutation of a device-resident batch (no counterpart in
+ * the reference, which has no gradients; the operator sits beside its CMA and
+ * MAP-Elites mutations): creature c's genes (per muscle the rhythm period and
+ * phase, contracted and extended length and strength; per node the friction,
+ * as src/physics2.rs:233-262 and src/kernel.rs:376-408 build the records from
+ * them) each take a step of relative size `rate` along d_grads[c *
+ * grad_stride ...] (evo_adjoint_device's layout), scaled per gene by the
+ * gene's magnitude and normalised by the creature's largest step, then are
+ * kept in range (period [0.5, 2] s, phase mod 1, strength [0.1, 1], friction
+ * [0.2, 1], contracted <= extended) and written back to the records in place.
+ * A row with a component beyond float range moves nothing but still passes
+ * the range step. Creatures [0, count) of the packed arrays; asynchronous on
+ * `stream`. */
+int evo_ascend_device(uint32_t* d_records, float* d_muscles, const uint32_t* d_heads, const float* d_grads,
+                      uint32_t count, uint32_t grad_stride, double rate, void* stream);
+
 /* Archive metrics and niche cells of creatures [base, base + count) from
  * their results, on the device: scheduler::to_metrics and trial_steps
  * (src/scheduler.rs:859-923), then qd::descriptor and Descriptor::niche_in

@@ -16,6 +16,7 @@ import oracle
 from evolution_simulator_gpu_amd import adjoint as A
 from evolution_simulator_gpu_amd import _native as N
 from evolution_simulator_gpu_amd import genome as G
+from evolution_simulator_gpu_amd.batch import concat
 
 WORDS = [8, 9, 10, 3, 4, 5]  # muscle record words in gradient order
 
@@ -431,3 +432,37 @@ def test_gpu_many_muscle_adjoint_matches_the_dual_oracle(gpu, kind):
         assert err < 1e-3, (kind, c, err)
         checked += 1
     assert checked >= 6
+
+
+@pytest.mark.gpu
+def test_gpu_ascend_is_the_host_mutation_bit_for_bit(gpu):
+    """The device mutation (evo_ascend_device) against ascend_batch on the
+    same gradients: random rows over many magnitudes, zero rows, rows with
+    inf and NaN components (moved as zero rows), bodies with and without
+    tendons and up to 64 muscles."""
+    import torch
+
+    rng = np.random.default_rng(21)
+    spec = E.synth_spec(4, 3000)
+    spec.max_nodes = 12
+    batch = E.synth(spec)
+    cs = [G.random_creature(rng, int(rng.integers(3, 13)), int(rng.integers(1, 61)), id=i) for i in range(200)]
+    batch = concat([batch, G.pack(cs, E.world())])
+    grads = (rng.normal(size=(batch.count, A.STRIDE)) * np.exp(rng.normal(size=(batch.count, 1)) * 4)).astype(np.float32)
+    h = batch.heads
+    own = 6 * h[:, 1].astype(np.int64) + (h[:, 0] & 255).astype(np.int64)
+    grads[np.arange(A.STRIDE)[None, :] >= own[:, None]] = 0.0
+    grads[::11] = 0.0
+    grads[5, 3] = np.inf
+    grads[17, 0] = np.nan
+    grads[23, own[23] - 1] = -np.inf
+    finite = np.isfinite(grads).all(axis=1)
+    want = A.ascend_batch(batch, np.where(finite[:, None], grads, 0.0), rate=0.02)
+    db = E.DeviceBatch(batch)
+    A.ascend_device(db, torch.from_numpy(grads.reshape(-1)).cuda(), rate=0.02)
+    torch.cuda.synchronize()
+    got_rec = db.records.cpu().numpy().view(np.uint32)
+    got_mus = db.muscles.cpu().numpy()
+    assert np.array_equal(got_rec, want.records)
+    assert np.array_equal(got_mus.view(np.uint32), want.muscles.view(np.uint32))
+    assert not np.array_equal(got_mus.view(np.uint32), batch.muscles.view(np.uint32))
