@@ -393,7 +393,8 @@ def full(args, torch):
             A.ascend_device(pair.db, pair.grads, rate=0.02)
         torch.cuda.synchronize()
         mutate_s = time.perf_counter() - t2
-        # replay capture of the fittest
+        # replay capture of the fittest (beside the pairs on a side stream it
+        # only takes their CUs: the generation is no shorter)
         e = [ev() for _ in range(2)]
         e[0].record()
         pick, frames, replay_res = gen.db.capture_top(replays, p, run)
@@ -409,6 +410,10 @@ def full(args, torch):
     same = True
     rel32 = []
     rng = np.random.default_rng(12)
+    # the replays (recording instance, any world) score as the trials did
+    replays_same = bool(np.array_equal(
+        replay_res.cpu().numpy().view(np.uint32),
+        res.view(np.float32).reshape(-1, 19)[pick.cpu().numpy()].view(np.uint32).reshape(-1)))
     grads = [pair.gradient_numpy() for pair in pairs]
     finite = [np.isfinite(g).all(axis=1) for g in grads]
     # the device mutation of the last generation against the host one
@@ -473,6 +478,7 @@ def full(args, torch):
             "device_mutation_equals_host": mutated_same,
             "replays": replays,
             "replay_ms": replay_ms,
+            "replays_match_trials": replays_same,
             "elite_gradients_vs_fp32_dual_oracle": {"sample": int(rel32.size),
                                                     "median_rel": finite_or_none(np.median(rel32)),
                                                     "within_1e-3": float((rel32 < 1e-3).mean())},

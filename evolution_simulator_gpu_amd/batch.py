@@ -47,23 +47,25 @@ class Batch:
             raise ValueError(f"creature {bad.value} of the batch cannot be simulated")
 
     def subset(self, index: np.ndarray) -> "Batch":
-        """The creatures at `index` (packed positions) as a batch of their own."""
-        index = np.asarray(index, dtype=np.int64)
-        recs, muss, heads = [], [], []
-        rw = mf = 0
-        for c in index:
-            h = self.heads[c].copy()
-            n, m = int(h[0] & 255), int(h[1])
-            words = n * N.NODE_WORDS + (n - 1) * N.BONE_WORDS
-            recs.append(self.records[h[3]:h[3] + words])
-            muss.append(self.muscles[h[4]:h[4] + m * N.MUSCLE_WORDS])
-            h[3], h[4] = rw, mf
-            rw += words
-            mf += m * N.MUSCLE_WORDS
-            heads.append(h)
-        return Batch(np.concatenate(recs).astype(np.uint32) if recs else np.zeros(1, np.uint32),
-                     np.concatenate(muss).astype(np.float32) if mf else np.zeros(4, np.float32),
-                     np.stack(heads).astype(np.uint32), self.slots[index].astype(np.uint32))
+        """The creatures at `index` (packed positions) as a batch of their own
+        (evo_subset: host threads copying each creature's records)."""
+        index = np.ascontiguousarray(index, dtype=np.uint32)
+        h = self.heads[index]
+        n = (h[:, 0] & 255).astype(np.int64)
+        words = int((n * N.NODE_WORDS + (n - 1) * N.BONE_WORDS).sum())
+        floats = int(h[:, 1].astype(np.int64).sum()) * N.MUSCLE_WORDS
+        records = np.empty(max(words, 1), np.uint32)
+        muscles = np.empty(max(floats, 4), np.float32)
+        heads = np.empty((index.size, 8), np.uint32)
+        if words == 0:
+            records[:] = 0
+        if floats == 0:
+            muscles[:] = 0.0
+        rc = N.lib().evo_subset(N.ptr(self.records), N.ptr(self.muscles), N.ptr(self.heads), N.ptr(index),
+                                index.size, N.ptr(records), records.size, N.ptr(muscles), muscles.size, N.ptr(heads))
+        if rc != N.EVO_OK:
+            raise ValueError("subset failed")
+        return Batch(records, muscles, heads, self.slots[index.astype(np.int64)].astype(np.uint32))
 
 
 def world(**overrides) -> N.WorldConfig:

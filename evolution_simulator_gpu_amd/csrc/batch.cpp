@@ -3,6 +3,9 @@
 // this library may come from anywhere, so every index the kernel follows is
 // checked here, and the kernel itself trusts the batch.
 #include <algorithm>
+#include <cstring>
+#include <thread>
+#include <vector>
 
 #include "host_common.hpp"
 
@@ -46,6 +49,41 @@ int evo_validate_batch(const uint32_t* records, uint64_t record_words, const flo
             if (sensor != EVO_NONE && sensor >= nodes) { return fail(); }
         }
     }
+    return EVO_OK;
+}
+
+int evo_subset(const uint32_t* records, const float* muscles, const uint32_t* heads, const uint32_t* index,
+               uint32_t count, uint32_t* out_records, uint64_t out_record_words, float* out_muscles,
+               uint64_t out_muscle_floats, uint32_t* out_heads) {
+    if ((count > 0u && (!records || !muscles || !heads || !index || !out_heads)) ) { return EVO_ERR_INVALID; }
+    // where each picked creature's records and muscle records go
+    std::vector<uint64_t> rec_at(count + 1u), mus_at(count + 1u);
+    for (uint32_t i = 0; i < count; i++) {
+        const uint32_t* h = heads + 8ull * index[i];
+        const uint64_t nodes = h[0] & 255u;
+        rec_at[i + 1u] = rec_at[i] + nodes * EVO_NODE_WORDS + (nodes - 1u) * EVO_BONE_WORDS;
+        mus_at[i + 1u] = mus_at[i] + static_cast<uint64_t>(h[1]) * EVO_MUSCLE_WORDS;
+    }
+    if (rec_at[count] > out_record_words || mus_at[count] > out_muscle_floats) { return EVO_ERR_INVALID; }
+    if (rec_at[count] > 0xffffffffull || mus_at[count] > 0xffffffffull) { return EVO_ERR_INVALID; }
+    const unsigned threads = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> pool;
+    for (unsigned t = 0; t < threads; t++) {
+        pool.emplace_back([&, t]() {
+            const uint32_t lo = static_cast<uint32_t>(static_cast<uint64_t>(count) * t / threads);
+            const uint32_t hi = static_cast<uint32_t>(static_cast<uint64_t>(count) * (t + 1u) / threads);
+            for (uint32_t i = lo; i < hi; i++) {
+                const uint32_t* h = heads + 8ull * index[i];
+                uint32_t* o = out_heads + 8ull * i;
+                std::memcpy(o, h, 8u * sizeof(uint32_t));
+                std::memcpy(out_records + rec_at[i], records + h[3], (rec_at[i + 1u] - rec_at[i]) * sizeof(uint32_t));
+                std::memcpy(out_muscles + mus_at[i], muscles + h[4], (mus_at[i + 1u] - mus_at[i]) * sizeof(float));
+                o[3] = static_cast<uint32_t>(rec_at[i]);
+                o[4] = static_cast<uint32_t>(mus_at[i]);
+            }
+        });
+    }
+    for (auto& th : pool) { th.join(); }
     return EVO_OK;
 }
 
