@@ -461,21 +461,31 @@ def from_bodies(bodies: list[dict], ids: list[int] | None = None, flags: list[in
                  np.arange(len(bodies), dtype=np.uint32))
 
 
-def class_ranges(batch: Batch, grids: tuple[int, ...] = (0, 0, 0)) -> list[tuple[int, int, int, int, int, int]]:
-    """The body-size classes of a class-major packed batch (synthetic spec
-    sorted = 2): bodies of at most 12 nodes and 24 muscles, bodies of at most
-    12 nodes and more muscles, then the rest, as (base, count, max_nodes,
-    max_muscles, no_tendons, grid) for each class the batch holds; `grids[k]`
-    is class k's grid (0: the kernel's own)."""
+def body_classes(nodes: np.ndarray, muscles: np.ndarray) -> np.ndarray:
+    """The body-size class of each creature: 0 at most 12 nodes and 24
+    muscles, 1 at most 12 nodes, 2 at most 20 nodes, 3 more."""
+    n = np.asarray(nodes).astype(np.int64)
+    m = np.asarray(muscles).astype(np.int64)
+    return np.where(n > 20, 3, np.where(n > 12, 2, np.where(m > 24, 1, 0)))
+
+
+CLASSES = 4
+
+
+def class_ranges(batch: Batch, grids: tuple[int, ...] = (0, 0, 0, 0)) -> list[tuple[int, int, int, int, int, int]]:
+    """The body-size classes (body_classes) of a class-major packed batch
+    (synthetic spec sorted = 2) as (base, count, max_nodes, max_muscles,
+    no_tendons, grid) for each class the batch holds; `grids[k]` is class k's
+    grid (0: the kernel's own)."""
     n = batch.nodes().astype(np.int64)
     m = batch.muscle_counts().astype(np.int64)
-    cls = np.where(n > 12, 2, np.where(m > 24, 1, 0))
+    cls = body_classes(n, m)
     if np.any(np.diff(cls) < 0):
         raise ValueError("the batch is not packed class-major")
-    cuts = [0] + [int(np.searchsorted(cls, k)) for k in (1, 2)] + [batch.count]
-    grids = tuple(grids) + (0,) * (3 - len(grids))
+    cuts = [0] + [int(np.searchsorted(cls, k)) for k in range(1, CLASSES)] + [batch.count]
+    grids = tuple(grids) + (0,) * (CLASSES - len(grids))
     out = []
-    for k in range(3):
+    for k in range(CLASSES):
         a, b = cuts[k], cuts[k + 1]
         if b > a:
             # a range's muscle records are contiguous in packed order

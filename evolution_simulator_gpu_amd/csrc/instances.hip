@@ -90,6 +90,12 @@ Instance pick_size(const evo_run& run) {
         }
         return instance<Cfg<R, S, WORLD, kMidN, EVO_MAX_MUSCLES, false, 64, true, true, false, true>>();
     }
+    // larger bodies run every loop at run time; up to 20 nodes their LDS
+    // rows leave room for 5 blocks on a CU (the 33 rows of 32 nodes for 3)
+    if (known && run.max_nodes <= static_cast<uint32_t>(kLoopN)) {
+        if (run.no_tendons) { return instance<Cfg<R, S, WORLD, kLoopN, EVO_MAX_MUSCLES, false, 64, false, false>>(); }
+        return instance<Cfg<R, S, WORLD, kLoopN, EVO_MAX_MUSCLES, false, 64, false, true>>();
+    }
     // bodies without tendons skip the tendon words of every muscle
     if (run.no_tendons) { return instance<Cfg<R, S, WORLD, EVO_MAX_NODES, EVO_MAX_MUSCLES, false, 64, false, false>>(); }
     return instance<Cfg<R, S, WORLD, EVO_MAX_NODES, EVO_MAX_MUSCLES, false, 64, false, true>>();

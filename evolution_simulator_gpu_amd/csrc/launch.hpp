@@ -71,6 +71,8 @@ Instance pick_ckpt_r120_s16(const evo_run& run);
 // simulator's maximum.
 constexpr int kSmallN = 12, kSmallM = 8;
 constexpr int kMidN = 12, kMidM = 24;
+// the looped class for bodies of up to 20 nodes (21 LDS rows: 5 blocks per CU)
+constexpr int kLoopN = 20;
 
 }  // namespace dev
 }  // namespace evo

@@ -261,10 +261,13 @@ int evo_synth_fill(const evo_synth_spec* s, uint32_t* records, float* muscles, u
     if (s->sorted) {
         // src/kernel.rs:487-493 — muscles, then nodes, then index; with
         // sorted = 2 the body-size classes come first (at most 12 nodes and
-        // 24 muscles; at most 12 nodes; the rest), so that each class is one
-        // contiguous range.
+        // 24 muscles; at most 12 nodes; at most 20 nodes; the rest), so that
+        // each class is one contiguous range.
         const bool class_major = s->sorted == 2u;
-        auto cls = [&](uint32_t c) { return counts[c].nodes > 12u ? 2 : counts[c].muscles > 24u ? 1 : 0; };
+        auto cls = [&](uint32_t c) {
+            const Counts& k = counts[c];
+            return k.nodes > 20u ? 3 : k.nodes > 12u ? 2 : k.muscles > 24u ? 1 : 0;
+        };
         std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
             if (class_major && cls(a) != cls(b)) { return cls(a) < cls(b); }
             if (counts[a].muscles != counts[b].muscles) { return counts[a].muscles < counts[b].muscles; }

@@ -21,7 +21,7 @@ import time
 import numpy as np
 
 from . import _native as N
-from .batch import Batch, DeviceBatch, Graph, WorldTable, class_ranges, params_for, run_for
+from .batch import Batch, DeviceBatch, Graph, WorldTable, body_classes, class_ranges, params_for, run_for
 
 
 def _events(torch, n):
@@ -81,8 +81,10 @@ class IslandGeneration:
                 # shares of the GPU, each class's grid that share of the
                 # blocks of it that fit alone; all 0: every kernel as large
                 # as fits, one after another as blocks free up
-                shares = [(1 / 3, 1 / 3, 1 / 3), (1 / 4, 1 / 2, 1 / 4), (1 / 4, 5 / 8, 1 / 8), (1 / 8, 3 / 4, 1 / 8),
-                          (3 / 8, 1 / 2, 1 / 8), (1 / 8, 5 / 8, 1 / 4)]
+                k = len(caps)
+                shares = [tuple(1 / k for _ in caps)]
+                for big in range(k):
+                    shares.append(tuple(0.5 if i == big else 0.5 / (k - 1) for i in range(k)))
                 candidates = [tuple(0 for _ in caps)]
                 candidates += [tuple(max(1, int(f * c)) for f, c in zip(share, caps)) for share in shares]
         short = [N.WorldConfig.from_buffer_copy(w) for w in self.worlds]
@@ -143,8 +145,10 @@ class PerShapeGeneration:
         world_of = np.asarray(world_of)
         n = batch.nodes().astype(np.int64)
         m = batch.muscle_counts().astype(np.int64)
-        # the body-size classes of class_ranges
-        cls = np.where(n > 12, 2, np.where(m > 24, 1, 0)).astype(np.int64)
+        # the body-size classes of class_ranges, bodies over 12 nodes as one
+        # shape: a per-world launch of the few bodies over 20 nodes would be
+        # a latency-bound launch of its own
+        cls = np.minimum(body_classes(n, m), 2)
         # packed order within each (world, class) group is kept
         self.order = np.lexsort((np.arange(batch.count), cls, world_of))
         self.batch = batch.subset(self.order)

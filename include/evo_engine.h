@@ -141,9 +141,9 @@ typedef struct evo_synth_spec {
     float duty;
     float stiffness;
     float joint_lo, joint_hi; /* half ranges of the joints, radians */
-    uint32_t sorted;          /* 1: pack sorted by (muscles, nodes, index); 2: the same within three body-size
+    uint32_t sorted;          /* 1: pack sorted by (muscles, nodes, index); 2: the same within four body-size
                                  classes in this order: at most 12 nodes and 24 muscles; at most 12 nodes;
-                                 the rest */
+                                 at most 20 nodes; the rest */
     uint32_t id_base;         /* creature c has id id_base + c (quake hash) */
     /* Ground the start pose stands on (world of creature c: worlds[world_of[c]]
      * when world_of is given to evo_synth_fill, else `world`). */

@@ -112,11 +112,12 @@ def test_world_table_host_checks():
     with pytest.raises(ValueError):
         E.WorldTable([worlds[0], E.world(duration=3.0, substeps=8)], world_of % 2, batch.count, 0)
     assert table.flag_union & N.TERRAIN and table.flag_union & N.GROUND
+    batch, *_ = _island_batch(count=4000, n_worlds=4)
     ranges = E.class_ranges(batch)
-    assert len(ranges) == 3 and ranges[0][0] == 0
+    assert len(ranges) == 4 and ranges[0][0] == 0
     assert all(b[0] == a[0] + a[1] for a, b in zip(ranges, ranges[1:])) and sum(r[1] for r in ranges) == batch.count
     assert ranges[0][2] <= 12 and ranges[0][3] <= 24 and ranges[1][2] <= 12 and ranges[1][3] > 24
-    assert ranges[2][2] > 12
+    assert 12 < ranges[2][2] <= 20 and ranges[3][2] > 20
     n, m = batch.nodes(), batch.muscle_counts()
     for k, (base, count, *_rest) in enumerate(ranges):
         cut = slice(base, base + count)
