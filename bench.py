@@ -34,10 +34,10 @@ sys.path.insert(0, ROOT)
 # add/sub/mul/div/min/max/floor/compare = 1, FMA = 2, sqrt = 1, cos = 14,
 # atan2 = 32; operations on launch constants are free.
 FLOPS_PER_CREATURE_SUBSTEP = 2148.7
-# Bytes the forward launch of this workload moves beyond L2, from rocprofv3
-# FETCH_SIZE and WRITE_SIZE scaled by a 256 MiB copy (reads x 2.0, writes x 1.0);
-# profiles/advance_kernel_plain_forward_3M.md gives the runs.
-TRAFFIC_BYTES_PER_LAUNCH = 1.578e12
+# Bytes the forward of this workload (its three class kernels) moves beyond L2,
+# from rocprofv3 FETCH_SIZE and WRITE_SIZE scaled by a 256 MiB copy (reads x 2.0,
+# writes x 1.0); profiles/advance_kernel_plain_forward_3M.md gives the runs.
+TRAFFIC_BYTES_PER_LAUNCH = 1.504e12
 FP32_PEAK_TFLOPS = 157.3  # MI355X dense FP32, 256 CUs at 2.4 GHz
 
 
@@ -531,9 +531,10 @@ def main():
     stream = torch.cuda.current_stream()
 
     def step(marks=None):
-        # one generation: the forward trials, the archive metrics and niche
+        # one generation: the forward trials (one persistent kernel per
+        # body-size class, run side by side), the archive metrics and niche
         # cells of every creature, and the replay capture of the fittest
-        db.launch(p, run, stream)
+        db.launch_classes(p, run, stream)
         if marks is not None:
             marks[0].record(stream)
         db.describe(p.steps, world.rate, N.LAYOUT_GLOBAL, stream, out=metrics)
@@ -611,9 +612,13 @@ def main():
             "unit": "TFLOP/s",
             "frac": achieved / FP32_PEAK_TFLOPS,
             "traffic": TRAFFIC_BYTES_PER_LAUNCH if args.creatures == 3_000_000 and args.duration == 20.0 else None,
-            "traffic_unit": "B per launch (rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0)",
+            "traffic_unit": "B per forward (rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, summed over the "
+                            "class kernels)",
             "flops_per_creature_substep": FLOPS_PER_CREATURE_SUBSTEP,
             "avg_launch_ms": avg_launch_s * 1e3,
+            "kernels": "advance_kernel, one per muscle-count class (12 nodes with 8, 16 and 24 muscles) forked "
+                       "onto side streams; achieved over their span, from HIP events on the launch stream "
+                       "around the fork and join",
         },
         "cpu_baseline": cpu,
     }

@@ -200,7 +200,7 @@ def run_for(batch: Batch, w: N.WorldConfig, record: bool = False) -> tuple[N.Par
 
 
 # Muscle counts at which the kernel's body-size classes change (csrc/launch.hpp).
-CLASS_MUSCLES = (8, 24)
+CLASS_MUSCLES = (8, 16, 24)
 
 
 def launch_ranges(batch: Batch) -> list[tuple[int, int, int, int]]:
@@ -261,6 +261,19 @@ class DeviceBatch:
                 self.frames.data_ptr() if self.frames is not None else None, C.c_void_p(stream.cuda_stream))
             if rc != N.EVO_OK:
                 raise RuntimeError(f"evo_advance_device failed: {rc}")
+
+    def launch_classes(self, p: N.Params, run: N.Run, stream=None) -> None:
+        """Simulates the whole batch as one persistent kernel per body-size
+        class (launch_ranges over the muscle-count classes of a batch packed
+        in (muscles, nodes) order), the kernels forked from and joined back
+        into `stream` so that one's tail overlaps the next one's start. A batch
+        in another order runs as one launch."""
+        m = self.batch.muscle_counts()
+        if run.record or p.base != 0 or p.count != self.batch.count or np.any(np.diff(m.astype(np.int64)) < 0):
+            self.launch(p, run, stream)
+            return
+        ranges = [(b, c, n, mm, int(run.no_tendons), 0) for b, c, n, mm in launch_ranges(self.batch)]
+        self.launch_ranges(p, run, ranges, worlds=False, stream=stream)
 
     def set_worlds(self, table: WorldTable) -> None:
         """Uploads a world table: every later launch_worlds / launch_ranges

@@ -70,6 +70,9 @@ Instance pick_unrolled(const evo_run& run) {
         if (run.max_nodes <= static_cast<uint32_t>(kSmallN) && run.max_muscles <= static_cast<uint32_t>(kSmallM)) {
             return instance<Cfg<R, S, WORLD, kSmallN, kSmallM, false, 128, true, TENDON>>();
         }
+        if (run.max_nodes <= static_cast<uint32_t>(kMidN) && run.max_muscles <= static_cast<uint32_t>(kMedM)) {
+            return instance<Cfg<R, S, WORLD, kMidN, kMedM, false, 128, true, TENDON>>();
+        }
     }
     return instance<Cfg<R, S, WORLD, kMidN, kMidM, false, 128, true, TENDON>>();
 }
