@@ -59,7 +59,7 @@ int launch(const Instance& in, const uint32_t* d_records, const float* d_muscles
 
 // Side streams for the concurrent ranges of one generation, created once per
 // device at the default priority.
-constexpr int kSideStreams = 4;
+constexpr int kSideStreams = 8;
 struct Side {
     hipStream_t stream[kSideStreams] = {};
     hipEvent_t fork = nullptr;

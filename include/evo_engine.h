@@ -212,7 +212,7 @@ typedef struct evo_class_range {
 } evo_class_range;
 
 /* One generation as concurrent persistent kernels, one per range (at most
- * 4), each claiming creatures of its range from its own counter
+ * 8), each claiming creatures of its range from its own counter
  * d_counters[i]. The ranges fork from `stream` onto internal streams and
  * join back into it, so work queued on `stream` afterwards sees every
  * result; the call can be captured into a graph (evo_graph_begin). With
