@@ -152,6 +152,36 @@ def test_random_creatures_pack_once_in_order():
     assert np.array_equal(again.records, batch.records) and np.array_equal(again.heads, batch.heads)
 
 
+def test_subset_copies_each_creature_and_renumbers_offsets():
+    """Batch.subset (evo_subset): each picked creature's node, bone and
+    muscle records land word for word at offsets that follow the pick order;
+    the other head words and the slots come along; an empty pick is an empty
+    batch; a pick of the whole batch in order is the batch."""
+    rng = np.random.default_rng(7)
+    batch = E.synth(E.synth_spec(4, 5000))
+    for pick in (rng.choice(batch.count, 1200, replace=False), np.sort(rng.choice(batch.count, 700, replace=False)),
+                 np.arange(batch.count), np.array([17]), np.array([], np.int64)):
+        sub = batch.subset(pick)
+        assert sub.count == len(pick)
+        rw = mf = 0
+        for i, c in enumerate(pick):
+            h = batch.heads[c]
+            n, m = int(h[0] & 255), int(h[1])
+            words = n * N.NODE_WORDS + (n - 1) * N.BONE_WORDS
+            assert np.array_equal(sub.records[rw:rw + words], batch.records[h[3]:h[3] + words])
+            got = sub.muscles[mf:mf + m * N.MUSCLE_WORDS].view(np.uint32)
+            assert np.array_equal(got, batch.muscles[h[4]:h[4] + m * N.MUSCLE_WORDS].view(np.uint32))
+            assert sub.heads[i, 3] == rw and sub.heads[i, 4] == mf
+            assert np.array_equal(np.delete(sub.heads[i], [3, 4]), np.delete(h, [3, 4]))
+            rw += words
+            mf += m * N.MUSCLE_WORDS
+        assert np.array_equal(sub.slots, batch.slots[pick.astype(np.int64)])
+        if len(pick):
+            sub.validate()
+    whole = batch.subset(np.arange(batch.count))
+    assert np.array_equal(whole.records, batch.records) and np.array_equal(whole.heads, batch.heads)
+
+
 def test_non_canonical_genomes_are_refused():
     w = _walker()
     bad = G.Creature(w.nodes, [dict(b) for b in w.bones], w.muscles)
