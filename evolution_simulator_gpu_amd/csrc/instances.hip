@@ -124,6 +124,15 @@ Instance EVO_PICK_CKPT(const evo_run& run) {
         return run.no_tendons ? instance<Cfg<R, S, kAnyWorld, kMidN, M, false, 64, true, false, true, true>>()
                               : instance<Cfg<R, S, kAnyWorld, kMidN, M, false, 64, true, true, true, true>>();
     }
+    if (run.max_muscles <= static_cast<uint32_t>(kMedM)) {
+        // the plain forward's 16-muscle class, with checkpoints
+        if (run.flags == kFlat) {
+            return run.no_tendons ? instance<Cfg<R, S, kFlat, kMidN, kMedM, false, 128, true, false, true>>()
+                                  : instance<Cfg<R, S, kFlat, kMidN, kMedM, false, 128, true, true, true>>();
+        }
+        return run.no_tendons ? instance<Cfg<R, S, kAnyWorld, kMidN, kMedM, false, 128, true, false, true>>()
+                              : instance<Cfg<R, S, kAnyWorld, kMidN, kMedM, false, 128, true, true, true>>();
+    }
     if (run.flags == kFlat) {
         return run.no_tendons ? instance<Cfg<R, S, kFlat, kMidN, kMidM, false, 128, true, false, true>>()
                               : instance<Cfg<R, S, kFlat, kMidN, kMidM, false, 128, true, true, true>>();
