@@ -188,7 +188,7 @@ SIGNATURES = {
     "evo_splitmix_finalize": ([_u64], _u64),
     "evo_device_error": ([], C.c_char_p),
     "evo_ascend_device": ([_P, _P, _P, _P, _u32, _u32, C.c_double, _P], C.c_int),
-    "evo_subset": ([_P, _P, _P, _P, _u32, _P, _u64, _P, _u64, _P], C.c_int),
+    "evo_subset": ([_P, _P, _P, _u32, _P, _u32, _P, _u64, _P, _u64, _P], C.c_int),
     "evo_adjoint_workspace_bytes": ([C.POINTER(Run), _u32, _u32, _u32], _u64),
     "evo_adjoint_forward_device": ([_P, _P, _P, _P, _P, C.POINTER(Params), C.POINTER(Run), _u32, _P, _u64, _P],
                                    C.c_int),

@@ -61,8 +61,9 @@ class Batch:
             records[:] = 0
         if floats == 0:
             muscles[:] = 0.0
-        rc = N.lib().evo_subset(N.ptr(self.records), N.ptr(self.muscles), N.ptr(self.heads), N.ptr(index),
-                                index.size, N.ptr(records), records.size, N.ptr(muscles), muscles.size, N.ptr(heads))
+        rc = N.lib().evo_subset(N.ptr(self.records), N.ptr(self.muscles), N.ptr(self.heads), self.count,
+                                N.ptr(index), index.size, N.ptr(records), records.size, N.ptr(muscles), muscles.size,
+                                N.ptr(heads))
         if rc != N.EVO_OK:
             raise ValueError("subset failed")
         return Batch(records, muscles, heads, self.slots[index.astype(np.int64)].astype(np.uint32))

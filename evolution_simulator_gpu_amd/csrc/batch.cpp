@@ -52,10 +52,13 @@ int evo_validate_batch(const uint32_t* records, uint64_t record_words, const flo
     return EVO_OK;
 }
 
-int evo_subset(const uint32_t* records, const float* muscles, const uint32_t* heads, const uint32_t* index,
-               uint32_t count, uint32_t* out_records, uint64_t out_record_words, float* out_muscles,
-               uint64_t out_muscle_floats, uint32_t* out_heads) {
-    if ((count > 0u && (!records || !muscles || !heads || !index || !out_heads)) ) { return EVO_ERR_INVALID; }
+int evo_subset(const uint32_t* records, const float* muscles, const uint32_t* heads, uint32_t creatures,
+               const uint32_t* index, uint32_t count, uint32_t* out_records, uint64_t out_record_words,
+               float* out_muscles, uint64_t out_muscle_floats, uint32_t* out_heads) {
+    if (count > 0u && (!records || !muscles || !heads || !index || !out_heads)) { return EVO_ERR_INVALID; }
+    for (uint32_t i = 0; i < count; i++) {
+        if (index[i] >= creatures) { return EVO_ERR_INVALID; }
+    }
     // where each picked creature's records and muscle records go
     std::vector<uint64_t> rec_at(count + 1u), mus_at(count + 1u);
     for (uint32_t i = 0; i < count; i++) {

@@ -465,13 +465,15 @@ const char* evo_last_error(const evo_engine* e);
 int evo_validate_batch(const uint32_t* records, uint64_t record_words, const float* muscles, uint64_t muscle_floats,
                        const uint32_t* heads, uint32_t creatures, uint32_t* bad);
 
-/* The creatures at index[0 .. count) of a packed batch as a packed batch of
- * their own, in that order (kernel::pack_reusing's layout, src/kernel.rs:471-593,
- * with the offsets of the heads renumbered): out_records and out_muscles
- * must hold the picked creatures' records; EVO_ERR_INVALID if they do not. */
-int evo_subset(const uint32_t* records, const float* muscles, const uint32_t* heads, const uint32_t* index,
-               uint32_t count, uint32_t* out_records, uint64_t out_record_words, float* out_muscles,
-               uint64_t out_muscle_floats, uint32_t* out_heads);
+/* The creatures at index[0 .. count) of a packed batch of `creatures`
+ * creatures as a packed batch of their own, in that order
+ * (kernel::pack_reusing's layout, src/kernel.rs:471-593, with the offsets of
+ * the heads renumbered): out_records and out_muscles must hold the picked
+ * creatures' records; EVO_ERR_INVALID if they do not or an index is out of
+ * range. */
+int evo_subset(const uint32_t* records, const float* muscles, const uint32_t* heads, uint32_t creatures,
+               const uint32_t* index, uint32_t count, uint32_t* out_records, uint64_t out_record_words,
+               float* out_muscles, uint64_t out_muscle_floats, uint32_t* out_heads);
 
 /* Largest muscle count of creatures [base, base + count) (for frame strides,
  * src/creature_kernel.rs:323-326). */

@@ -180,6 +180,12 @@ def test_subset_copies_each_creature_and_renumbers_offsets():
             sub.validate()
     whole = batch.subset(np.arange(batch.count))
     assert np.array_equal(whole.records, batch.records) and np.array_equal(whole.heads, batch.heads)
+    # the C entry point refuses an index past the batch
+    out = np.zeros(1024, np.uint32)
+    bad = np.array([batch.count], np.uint32)
+    assert N.lib().evo_subset(N.ptr(batch.records), N.ptr(batch.muscles), N.ptr(batch.heads), batch.count,
+                              N.ptr(bad), 1, N.ptr(out), out.size, N.ptr(out.view(np.float32)), out.size,
+                              N.ptr(out)) == N.EVO_ERR_INVALID
 
 
 def test_non_canonical_genomes_are_refused():
