@@ -296,6 +296,11 @@ struct Body {
     float maa[C::MAXM], mab[C::MAXM], mhill[C::MAXM], mcap[C::MAXM], minvcap[C::MAXM];
     float mtk[C::tendon ? C::MAXM : 1], mslack[C::tendon ? C::MAXM : 1];
     float mstore[C::MAXM], mdemand[C::MAXM];
+    // the rhythm lines (amplitude, inverse period, phase, duty; inverse duty,
+    // inverse complement, stiffness, reset) where muscles run as run-time
+    // loops: per-lane scratch rows read by whole waves at once, instead of
+    // every lane reading its own record every step
+    float4 mr2[C::munrolled ? 1 : C::MAXM], mr3[C::munrolled ? 1 : C::MAXM];
     float mpull[C::record ? C::MAXM : 1];
     float normal_force[C::record ? C::MAXN : 1], friction_force[C::record ? C::MAXN : 1];
 };
@@ -437,8 +442,15 @@ __device__ __forceinline__ void muscle_demands(Body<C>& b, const Cols<C>& s, con
     mitems<C, C::MAXM, 0, 1>(wb.m, [&](auto kc) {
         const int k = kc;
         if (static_cast<uint32_t>(k) < b.muscles) {
-            const float4* q = reinterpret_cast<const float4*>(mus + k * EVO_MUSCLE_WORDS);
-            const float4 r2 = q[2], r3 = q[3];
+            float4 r2, r3;
+            if constexpr (C::munrolled) {
+                const float4* q = reinterpret_cast<const float4*>(mus + k * EVO_MUSCLE_WORDS);
+                r2 = q[2];
+                r3 = q[3];
+            } else {
+                r2 = b.mr2[k];
+                r3 = b.mr3[k];
+            }
             const float offset = s.o(k);
             const float before = rhythm(t, r2.y, r2.z, offset, r2.w, r3.x, r3.y);
             const float after = rhythm(t + C::dt, r2.y, r2.z, offset, r2.w, r3.x, r3.y);
@@ -876,6 +888,10 @@ __device__ __forceinline__ void begin_creature(Body<C>& b, Trial& tr, const Cols
         if (static_cast<uint32_t>(k) < b.muscles) {
             const float4* q = reinterpret_cast<const float4*>(muscles + tr.mus_off + k * EVO_MUSCLE_WORDS);
             const float4 r0 = q[0], r1 = q[1];
+            if constexpr (!C::munrolled) {
+                b.mr2[k] = q[2];
+                b.mr3[k] = q[3];
+            }
             b.mnodes[k] = f_bits(r0.x);
             b.maa[k] = r0.y;
             b.mab[k] = r0.z;
@@ -897,6 +913,10 @@ __device__ __forceinline__ void begin_creature(Body<C>& b, Trial& tr, const Cols
             if constexpr (C::tendon) {
                 b.mtk[k] = 0.0f;
                 b.mslack[k] = 0.0f;
+            }
+            if constexpr (!C::munrolled) {
+                b.mr2[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                b.mr3[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             }
         }
         b.mstore[k] = 1.0f;
