@@ -43,6 +43,10 @@ class IslandGeneration:
         self.db.set_worlds(self.table)
         self.p, self.run = self.table.run_for(batch)
         self.ranges = class_ranges(batch)
+        # one world for every creature: the world's parameters go to the
+        # kernels as launch arguments (the any-world instances), not through
+        # the table (1.13x on the full-stack batch, results identical)
+        self.table_worlds = len(worlds) > 1
         self.stream = torch.cuda.Stream()
         self.graph: Graph | None = None
         self.grids = tuple(0 for _ in self.ranges)
@@ -56,11 +60,12 @@ class IslandGeneration:
         for r in self.ranges:
             q = N.Run.from_buffer_copy(self.run)
             q.max_nodes, q.max_muscles, q.no_tendons = r[2], r[3], r[4]
-            out.append(int(N.lib().evo_kernel_capacity(C.byref(q), 1)))
+            out.append(int(N.lib().evo_kernel_capacity(C.byref(q), int(self.table_worlds))))
         return out
 
     def launch(self, grids=None, p: N.Params | None = None) -> None:
-        self.db.launch_ranges(p or self.p, self.run, self._ranges(grids or self.grids), stream=self.stream)
+        self.db.launch_ranges(p or self.p, self.run, self._ranges(grids or self.grids), worlds=self.table_worlds,
+                              stream=self.stream)
 
     def tune(self, seconds: float = 2.0, candidates=None, progress=None) -> dict:
         """Times each candidate split of the GPU between the classes on this
@@ -122,7 +127,8 @@ class IslandGeneration:
         self.launch()  # creates the side streams before the capture
         torch.cuda.synchronize()
         g = Graph()
-        g.capture(lambda s: self.db.launch_ranges(self.p, self.run, self._ranges(self.grids), stream=s), self.stream)
+        g.capture(lambda s: self.db.launch_ranges(self.p, self.run, self._ranges(self.grids),
+                                                  worlds=self.table_worlds, stream=s), self.stream)
         self.graph = g
 
     def step(self) -> None:

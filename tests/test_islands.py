@@ -172,3 +172,27 @@ def test_replayed_generation_graph_matches_eager(gpu):
         g.replay(side)
         torch.cuda.synchronize()
         assert np.array_equal(bits(db.results_numpy()), bits(eager)), f"replay {rep}"
+
+
+@pytest.mark.gpu
+def test_one_world_generation_runs_without_the_table(gpu):
+    """A generation whose creatures share one world passes the world as
+    launch arguments (no table): tuned, captured and replayed, its results
+    equal the plain simulator's bit for bit."""
+    import torch
+
+    from evolution_simulator_gpu_amd.generation import IslandGeneration
+
+    w = E.world(duration=1.0, substeps=8, terrain=0.1, gaps=0.35, patches=0.8)
+    spec = E.synth_spec(4, 3000, world=w)
+    batch = E.synth(spec)
+    gen = IslandGeneration(batch, [w], np.zeros(batch.count, np.uint32))
+    assert not gen.table_worlds
+    gen.tune(seconds=0.25, candidates=[tuple(0 for _ in gen.ranges)])
+    gen.capture()
+    gen.db.results.zero_()
+    gen.step()
+    torch.cuda.synchronize()
+    got = gen.results()
+    want = E.simulate(batch, w)
+    assert np.array_equal(bits(got), bits(want))
