@@ -494,7 +494,7 @@ def full(args, torch):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported on these boxes.",
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported.",
               file=sys.stderr)
         sys.exit(2)
     import torch
