@@ -11,8 +11,8 @@ The metric is creature-substeps per second: the substeps actually simulated
 (steps run x 8, summed over creatures, as examples/p2_speed.rs:56-72 counts
 them) divided by the timed wall seconds.
 
-Prints one JSON line. `--gpus N` with N > 1 is not supported: every box here
-has one GPU.
+Prints one JSON line. `--gpus N` with N > 1 is not supported: the benchmark
+measures one GPU.
 """
 from __future__ import annotations
 
