@@ -1297,6 +1297,11 @@ __global__ void __launch_bounds__(C::BLOCK) advance_kernel(const uint32_t* __res
     __shared__ float4 s_pv[(C::MAXN + 1) * C::BLOCK];
     __shared__ float2 s_fp[(C::MAXN + 1) * C::BLOCK];
     __shared__ float s_off[(C::munrolled && C::MAXM > 0 ? C::MAXM : 1) * C::BLOCK];
+    // per-creature worlds, unrolled classes: each lane's world parameters in
+    // LDS (the class's blocks still fit two to a CU), read there where used
+    // instead of from the world table through the L1
+    constexpr bool kStageWorld = C::world == kTableWorld && C::munrolled;
+    __shared__ evo_params s_wp[kStageWorld ? C::BLOCK : 1];
     float l_off[C::munrolled ? 1 : C::MAXM];
     const uint32_t lane = threadIdx.x & 63u;
     const uint64_t below = (1ull << lane) - 1ull;
@@ -1362,6 +1367,7 @@ __global__ void __launch_bounds__(C::BLOCK) advance_kernel(const uint32_t* __res
             }
             if constexpr (C::world == kTableWorld) {
                 tr.world = worlds.of[p.base + at];
+                if constexpr (kStageWorld) { s_wp[threadIdx.x] = worlds.params[tr.world]; }
                 begin_creature<C>(b, tr, s, p.base + at, records, muscles, heads, worlds.params[tr.world],
                                   worlds.flags[tr.world], my_frames);
             } else {
@@ -1380,7 +1386,7 @@ __global__ void __launch_bounds__(C::BLOCK) advance_kernel(const uint32_t* __res
                 // used (L1-resident table), not held in registers
                 uint32_t w = tr.world;
                 opaque<C>(w, wb);
-                const evo_params& pw = worlds.params[w];
+                const evo_params& pw = kStageWorld ? s_wp[threadIdx.x] : worlds.params[w];
                 const float rh = 0.5f * pw.muscle_recovery * C::h;
                 run_step<C>(b, tr, s, muscles, results, pw, worlds.flags[w], rh, my_frames, wb, ck);
             } else {
