@@ -78,9 +78,10 @@ __global__ void __launch_bounds__(256) ascend_kernel(uint32_t* __restrict__ reco
     uint32_t* rec = records + h0.w;
     float* mus = muscles + h1.x;
     const float* gr = grads + static_cast<uint64_t>(c) * stride;
-    // a gradient row with a component beyond float range moves nothing
-    bool finite = true;
-    for (uint32_t q = 0; q < 6u * m + n; q++) { finite = finite && isfinite(gr[q]); }
+    // a gradient row with a component beyond float range, or one that
+    // cannot hold the creature's parameters, moves nothing
+    bool finite = 6u * m + n <= stride;
+    for (uint32_t q = 0; finite && q < 6u * m + n; q++) { finite = isfinite(gr[q]); }
     const auto grad = [&](uint32_t q) { return finite ? static_cast<double>(gr[q]) : 0.0; };
 
     // the largest scaled step of each gene over the creature's items

@@ -297,8 +297,9 @@ int evo_adjoint_device(const uint32_t* d_records, const float* d_muscles, const 
  * gene's magnitude and normalised by the creature's largest step, then are
  * kept in range (period [0.5, 2] s, phase mod 1, strength [0.1, 1], friction
  * [0.2, 1], contracted <= extended) and written back to the records in place.
- * A row with a component beyond float range moves nothing but still passes
- * the range step. Creatures [0, count) of the packed arrays; asynchronous on
+ * A row with a component beyond float range, or a creature with more
+ * parameters than grad_stride holds, moves nothing but still passes the
+ * range step. Creatures [0, count) of the packed arrays; asynchronous on
  * `stream`. */
 int evo_ascend_device(uint32_t* d_records, float* d_muscles, const uint32_t* d_heads, const float* d_grads,
                       uint32_t count, uint32_t grad_stride, double rate, void* stream);
