@@ -92,3 +92,42 @@ def test_class_kernels_side_by_side(gpu):
     db.launch(p, run)
     torch.cuda.synchronize()
     assert_bit_exact(db.results_numpy(), want, "one launch")
+
+
+def test_bodies_of_13_to_20_nodes_use_the_20_node_class(gpu):
+    """Bodies of 13-20 nodes run the run-time-loop class sized for 20 nodes
+    (21 LDS rows): bit for bit the oracle's, calm and hard world, with and
+    without tendons."""
+    spec = E.synth_spec(2, 384)
+    spec.min_nodes, spec.max_nodes = 13, 20
+    batch = E.synth(spec)
+    assert batch.nodes().min() >= 13 and batch.nodes().max() <= 20
+    got, want = _both(batch, flat_world(duration=2.0))
+    assert_bit_exact(got, want, "13-20 nodes")
+    hard = E.world(duration=2.0, substeps=8, terrain=0.25, gravity=19.6, ground_friction=0.6, patches=0.8,
+                   gaps=0.35, hurdles=0.08, quake=0.05, water=0.12, wind=-1.0)
+    got, want = _both(batch, hard)
+    assert_bit_exact(got, want, "13-20 nodes, hard world")
+    rng = np.random.default_rng(17)
+    cs = [G.random_creature(rng, int(rng.integers(13, 21)), int(rng.integers(5, 40)), id=900 + i) for i in range(128)]
+    w = flat_world(duration=2.0)
+    tendons = G.pack(cs, w)
+    assert not tendons.tendon_free() and tendons.nodes().max() <= 20
+    got, want = _both(tendons, w)
+    assert_bit_exact(got, want, "13-20 nodes with tendons")
+
+
+def test_class_kernels_fall_back_to_one_launch_when_unsorted(gpu):
+    """launch_classes on a batch not in (muscles, nodes) order runs one
+    launch, with the same results."""
+    import torch
+
+    batch = E.synth(E.synth_spec(1, 2048))
+    shuffled = batch.subset(np.random.default_rng(4).permutation(batch.count))
+    w = flat_world(duration=1.0)
+    p, run = E.run_for(shuffled, w)
+    want, _, _ = oracle.run_f32(shuffled, p, run, threads=8)
+    db = E.DeviceBatch(shuffled)
+    db.launch_classes(p, run)
+    torch.cuda.synchronize()
+    assert_bit_exact(db.results_numpy(), want, "unsorted batch, class launch")
