@@ -233,7 +233,7 @@ const char* evo_device_error(void) { return hipGetErrorName(evo::dev::last_failu
 
 const char* evo_build_info(void) {
     return "evo-mi355x gfx950: rates {60,120} x substeps {8,16} x worlds {flat, any, per-creature table} x sizes "
-           "{12n/8m unrolled, 12n/16m unrolled, 12n/24m unrolled, 12n/96m muscles looped, 20n/96m looped, 32n/96m looped}; recording: any world, 12n/24m unrolled or looped";
+           "{12n/8m unrolled, 12n/16m unrolled, 12n/24m unrolled, 12n/96m muscles looped, 20n/96m looped, 32n/96m looped}; recording: 12n/24m unrolled (flat tendon-free or any world) or looped";
 }
 
 }  // extern "C"

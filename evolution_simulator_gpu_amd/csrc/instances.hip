@@ -148,6 +148,11 @@ Instance EVO_PICK(const evo_run& run, bool table) {
         // capture) run unrolled; larger or unknown bodies run the looped class
         if (run.max_nodes != 0u && run.max_nodes <= static_cast<uint32_t>(kMidN) &&
             run.max_muscles <= static_cast<uint32_t>(kMidM)) {
+            // flat ground without tendons (the plain forward's replays): a
+            // single wave's latency is the whole cost, so the leaner instance
+            if (run.flags == kFlat && run.no_tendons) {
+                return instance<Cfg<EVO_RATE, EVO_SUB, kFlat, kMidN, kMidM, true, 128, true, false>>();
+            }
             return instance<Cfg<EVO_RATE, EVO_SUB, kAnyWorld, kMidN, kMidM, true, 128, true, true>>();
         }
         return instance<Cfg<EVO_RATE, EVO_SUB, kAnyWorld, EVO_MAX_NODES, EVO_MAX_MUSCLES, true, 64, false, true>>();
