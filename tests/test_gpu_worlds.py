@@ -99,6 +99,25 @@ def test_recording_frames_and_score(gpu):
         assert np.array_equal(a[~nan], b[~nan])
 
 
+def test_recording_bodies_with_tendons_on_flat_ground(gpu):
+    """Flat ground with tendons records through the any-world unrolled
+    instance (flat tendon-free batches take their own): frames and result
+    equal the oracle's."""
+    from evolution_simulator_gpu_amd import genome as G
+
+    rng = np.random.default_rng(23)
+    w = E.world(duration=1.5, substeps=8)
+    cs = [G.random_creature(rng, int(rng.integers(3, 13)), int(rng.integers(2, 15)), id=300 + i) for i in range(24)]
+    batch = G.pack(cs, w)
+    assert not batch.tendon_free() and batch.nodes().max() <= 12 and batch.muscle_counts().max() <= 24
+    p, run = E.run_for(batch, w, record=True)
+    want, wframes, _ = oracle.run_f32(batch, p, run, threads=4, frames=True)
+    got, frames = E.simulate(batch, w, record=True)
+    assert_bit_exact(got, want, "recording with tendons, flat ground")
+    nan = np.isnan(frames) & np.isnan(wframes)
+    assert np.array_equal(frames.view(np.uint32)[~nan], wframes.view(np.uint32)[~nan])
+
+
 def test_repeat_runs_are_bit_identical(gpu):
     batch = E.synth(E.synth_spec(1, 4096))
     w = E.world(duration=3.0, substeps=8)
