@@ -28,7 +28,7 @@ namespace {
 template <class C>
 hipError_t launch(const AdjointArgs& x, hipStream_t stream) {
     const Regions r = regions<C::MAXN, C::MAXM>(x.workspace, x.stride, x.segments, x.seg, C::substeps);
-    const uint32_t blocks = (x.p.count + 63u) / 64u;
+    const uint32_t blocks = blocks_for(x.p.count, 64u);
     hipLaunchKernelGGL(adjoint_kernel<C>, dim3(blocks), dim3(64), 0, stream, x.records, x.muscles, x.heads,
                        x.results, x.p, x.flags, x.seg, x.grads, x.fitness, r.ckpt, r.slots, r.tapes, r.steps, r.cols,
                        x.stride, x.checkpointed);

@@ -151,7 +151,7 @@ int evo_ascend_device(uint32_t* d_records, float* d_muscles, const uint32_t* d_h
     if (!d_records || !d_muscles || !d_heads || !d_grads || grad_stride == 0u) { return EVO_ERR_INVALID; }
     if ((reinterpret_cast<uintptr_t>(d_heads) & 15u) != 0u) { return EVO_ERR_INVALID; }
     if (count == 0u) { return EVO_OK; }
-    const uint32_t blocks = (count + 255u) / 256u;
+    const uint32_t blocks = evo::dev::blocks_for(count, 256u);
     hipLaunchKernelGGL(ascend_kernel, dim3(blocks), dim3(256), 0, reinterpret_cast<hipStream_t>(stream), d_records,
                        d_muscles, reinterpret_cast<const uint4*>(d_heads), d_grads, count, grad_stride, rate);
     return evo::dev::device_code(hipGetLastError());

@@ -58,9 +58,13 @@ int evo_subset(const uint32_t* records, const float* muscles, const uint32_t* he
     if (count > 0u && (!records || !muscles || !heads || !index || !out_heads)) { return EVO_ERR_INVALID; }
     for (uint32_t i = 0; i < count; i++) {
         if (index[i] >= creatures) { return EVO_ERR_INVALID; }
+        // counts outside the simulator's limits would make the extents below wrap
+        const uint32_t* h = heads + 8ull * index[i];
+        const uint32_t nodes = h[0] & 255u;
+        if (nodes < 2u || nodes > EVO_MAX_NODES || h[1] > EVO_MAX_MUSCLES) { return EVO_ERR_INVALID; }
     }
     // where each picked creature's records and muscle records go
-    std::vector<uint64_t> rec_at(count + 1u), mus_at(count + 1u);
+    std::vector<uint64_t> rec_at(static_cast<size_t>(count) + 1u), mus_at(static_cast<size_t>(count) + 1u);
     for (uint32_t i = 0; i < count; i++) {
         const uint32_t* h = heads + 8ull * index[i];
         const uint64_t nodes = h[0] & 255u;

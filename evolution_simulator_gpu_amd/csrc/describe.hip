@@ -154,7 +154,7 @@ extern "C" int evo_describe_device(const uint32_t* d_records, const uint32_t* d_
     if (!d_records || !d_heads || !d_results || !d_out || rate == 0u) { return EVO_ERR_INVALID; }
     if (layout != EVO_LAYOUT_ISLAND && layout != EVO_LAYOUT_GLOBAL) { return EVO_ERR_INVALID; }
     if (count == 0u) { return EVO_OK; }
-    const uint32_t blocks = (count + 255u) / 256u;
+    const uint32_t blocks = evo::dev::blocks_for(count, 256u);
     hipLaunchKernelGGL(describe_kernel, dim3(blocks), dim3(256), 0, reinterpret_cast<hipStream_t>(stream), d_records,
                        reinterpret_cast<const uint4*>(d_heads), d_results, base, count, steps,
                        static_cast<float>(rate), layout, d_out);

@@ -15,6 +15,10 @@ namespace dev {
 // EVO_ERR_DEVICE); a failure is kept for evo_device_error.
 int device_code(hipError_t e);
 
+// Blocks of `per` threads that cover `count` items, without the wrap of
+// (count + per - 1) / per near the top of the 32-bit range.
+constexpr uint32_t blocks_for(uint32_t count, uint32_t per) { return count / per + (count % per != 0u ? 1u : 0u); }
+
 
 // The worlds of a batch whose creatures run in different worlds: world k's
 // launch parameters and effect bits, and the world of each packed creature.

@@ -470,8 +470,9 @@ int evo_validate_batch(const uint32_t* records, uint64_t record_words, const flo
  * creatures as a packed batch of their own, in that order
  * (kernel::pack_reusing's layout, src/kernel.rs:471-593, with the offsets of
  * the heads renumbered): out_records and out_muscles must hold the picked
- * creatures' records; EVO_ERR_INVALID if they do not or an index is out of
- * range. */
+ * creatures' records; EVO_ERR_INVALID if they do not, an index is out of
+ * range or a picked head's counts are outside the simulator's limits. The
+ * source offsets are trusted: pass a batch evo_validate_batch accepts. */
 int evo_subset(const uint32_t* records, const float* muscles, const uint32_t* heads, uint32_t creatures,
                const uint32_t* index, uint32_t count, uint32_t* out_records, uint64_t out_record_words,
                float* out_muscles, uint64_t out_muscle_floats, uint32_t* out_heads);

@@ -186,6 +186,14 @@ def test_subset_copies_each_creature_and_renumbers_offsets():
     assert N.lib().evo_subset(N.ptr(batch.records), N.ptr(batch.muscles), N.ptr(batch.heads), batch.count,
                               N.ptr(bad), 1, N.ptr(out), out.size, N.ptr(out.view(np.float32)), out.size,
                               N.ptr(out)) == N.EVO_ERR_INVALID
+    # and a picked head whose counts are outside the simulator's limits
+    one = np.array([0], np.uint32)
+    for word, value in [(0, 0), (0, 1), (0, 33), (1, 97)]:
+        heads = batch.heads.copy()
+        heads[0, word] = (heads[0, word] & ~np.uint32(255)) | value if word == 0 else value
+        assert N.lib().evo_subset(N.ptr(batch.records), N.ptr(batch.muscles), N.ptr(heads), batch.count,
+                                  N.ptr(one), 1, N.ptr(out), out.size, N.ptr(out.view(np.float32)), out.size,
+                                  N.ptr(out)) == N.EVO_ERR_INVALID, (word, value)
 
 
 def test_non_canonical_genomes_are_refused():
