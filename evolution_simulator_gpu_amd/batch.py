@@ -363,6 +363,12 @@ class DeviceBatch:
         q.base, q.count = 0, k
         r = N.Run.from_buffer_copy(run)
         r.record = 1
+        if k > 0:
+            # the recording instance for the picked bodies' own sizes (one
+            # small read-back): the fittest of a batch with large bodies may
+            # all fit the unrolled class
+            nodes, muscles = torch.stack([(heads[:, 0] & 255).max(), heads[:, 1].max()]).cpu().tolist()
+            r.max_nodes, r.max_muscles = nodes, muscles
         fpc = N.lib().evo_settle_frames(run.rate) + p.steps + 1
         # slots past a creature's own nodes and muscles are not written: zero them
         frames = torch.zeros(k * fpc * p.stride * 2, dtype=torch.float32, device=self.device)
