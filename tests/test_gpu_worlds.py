@@ -142,14 +142,17 @@ def test_a_creature_scores_the_same_in_any_batch(gpu):
     assert np.array_equal(bits(again), bits(together)[picks[::-1]])
 
 
-def test_top_k_replay_capture(gpu):
+@pytest.mark.parametrize("preset", [1, 2])
+def test_top_k_replay_capture(gpu, preset):
     """The k fittest creatures recorded in one launch through gathered head
     words: the replayed result equals the trial's result bit for bit
     (tests/gpu_repeatability.rs:76-140) and every frame equals the oracle's
-    recording of that creature."""
+    recording of that creature. The recording class follows the picked
+    bodies' sizes, so a batch of 3-32 node bodies replays its fittest in
+    whichever class they fit."""
     import torch
 
-    batch = E.synth(E.synth_spec(1, 3000))
+    batch = E.synth(E.synth_spec(preset, 3000))
     w = E.world(duration=1.5, substeps=8)
     p, run = E.run_for(batch, w)
     db = E.DeviceBatch(batch)
